@@ -1,0 +1,198 @@
+/*
+ * laph.h -- C-ABI of the MI355X (gfx950) LapH per-timeslice eigensystem
+ * library (liblapheig.so).
+ *
+ * Conventions
+ *   - extern "C", plain pointers and sizes, int return: 0 = LAPH_OK,
+ *     negative = error (laph_error_string / laph_last_error explain it).
+ *     No exception crosses this boundary.
+ *   - Complex numbers are interleaved (re, im) doubles (layout-compatible
+ *     with PetscScalar = double _Complex and std::complex<double>).
+ *   - Geometry: isotropic L^3 timeslice, V3 = L^3, site s = x*L*L + y*L + z,
+ *     dir 0/1/2 = x/y/z, vector index 3*s + colour (n = 3*V3 entries).
+ *   - Links (host side): U[((s*3 + dir)*9 + 3*a + b)] complex, row a, col b.
+ *   - Caller owns host buffers; the library owns every device buffer it
+ *     allocates.  One context per host thread; a context is not re-entrant.
+ *
+ * Each entry point names the reference interface it replaces
+ * (file:line in the LapH_EigSys sources).
+ */
+#ifndef LAPH_H
+#define LAPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LAPH_OK 0
+#define LAPH_ERR_INVALID (-1)  /* bad argument or size */
+#define LAPH_ERR_HIP (-2)      /* HIP runtime / library failure */
+#define LAPH_ERR_NOMEM (-3)    /* device or host allocation failed */
+#define LAPH_ERR_IO (-4)       /* file open / read / write failure */
+#define LAPH_ERR_FORMAT (-5)   /* LIME record missing or wrong size */
+#define LAPH_ERR_NOCONV (-6)   /* eigensolver stopped before nev pairs converged */
+#define LAPH_ERR_TRACE (-7)    /* tr(V^H V) guard failed */
+
+const char* laph_error_string(int code);
+/* Message of the last error raised on the calling thread ("" if none). */
+const char* laph_last_error(void);
+/* Library version string. */
+const char* laph_version(void);
+
+/* Parameters that ev_ts reads from parameters.txt (par_io.cpp:99-135):
+ * LX (= LY = LZ), DEG, LAM_L, LAM_C; plus the HIP device ordinal. */
+typedef struct laph_params {
+  int L;
+  int deg;
+  double lambda_l;
+  double lambda_c;
+  int device;
+} laph_params;
+
+typedef struct laph_ctx laph_ctx;
+
+/* Creates a context on params->device, uploads the spatial links of one
+ * timeslice given in the io.cpp config layout (72*V3 doubles, as produced by
+ * laph_read_lime), or unit links when ts_config is NULL.
+ * Replaces Tslice::init + Tslice::map_timeslice_to_eigen
+ * (timeslice.cpp:19-31, 34-79) and the MatCreateShell context
+ * (ev_ts.cpp:197). */
+int laph_ctx_create(laph_ctx** out, const laph_params* params, const double* ts_config);
+int laph_ctx_destroy(laph_ctx* ctx);
+
+/* Tslice::map_timeslice_to_eigen (timeslice.cpp:34-79). */
+int laph_map_timeslice(laph_ctx* ctx, const double* ts_config);
+/* Tslice::set_gauge / get_gauge for the whole timeslice (timeslice.cpp:81-88):
+ * V3*3*9 complex in the host link layout above. */
+int laph_set_links(laph_ctx* ctx, const double* links);
+int laph_get_links(laph_ctx* ctx, double* links);
+
+/* Tslice::smearing_hyp(alpha_1, alpha_2, iter) (timeslice.cpp:243-369),
+ * on the device links of ctx. */
+int laph_hyp_smear(laph_ctx* ctx, double alpha_1, double alpha_2, int iter);
+
+/* MatMult_Laplacian2D (shell_matop.cpp:226-244): y = T_DEG(B) x with
+ * B = c*D + a, c = 2/(LAM_L-LAM_C), a = 1 + c*LAM_C.  x, y: host arrays of
+ * n = 3*V3 complex.  ctx is the laph_ctx* stored as the MatShell context.
+ * DEG >= 1 (T_1(B) = B); DEG < 1 is rejected at context creation. */
+int laph_shell_mult(void* ctx, const double* x, double* y);
+/* MatGetDiagonal_Laplacian2D (shell_matop.cpp:248-255): diag[i] = -6 + 0i,
+ * the diagonal of D (the value the reference registers), n complex. */
+int laph_shell_get_diagonal(void* ctx, double* diag);
+/* Device-pointer variant of laph_shell_mult (n complex on the context's
+ * device; completes before returning). */
+int laph_shell_mult_device(laph_ctx* ctx, const double* dx, double* dy);
+/* y = D x (tv2 with c = 1, a = 0; shell_matop.cpp:68-77), host arrays. */
+int laph_laplacian(laph_ctx* ctx, const double* x, double* y);
+
+/* ------------------------------------------------------------------------
+ * EPS-shaped solve of the plain FP64 drop-in (ev_ts.cpp:231-303): Hermitian
+ * Krylov-Schur for the nev eigenvalues of T_DEG(B) of largest magnitude,
+ * with the search space, every Lanczos vector and the filtered matvec
+ * resident on the device.  Outputs: theta (nev doubles, |theta| descending,
+ * as EPSGetEigenpair returns them), V (host n x nev complex column-major,
+ * may be NULL). */
+typedef struct laph_eps_opts {
+  int nev;                 /* EPSSetDimensions nev */
+  int ncv;                 /* 0 = max(2*nev, nev+15) */
+  double tol;              /* -eps_tol: |r|/|theta| < tol */
+  int max_it;              /* 0 = max(100, 2n/ncv) */
+  unsigned long long seed; /* start vector */
+} laph_eps_opts;
+
+typedef struct laph_eps_stats {
+  int restarts;
+  int nconv;
+  long long matvecs; /* applications of T_DEG(B) */
+  double seconds;
+} laph_eps_stats;
+
+int laph_eps_solve(laph_ctx* ctx, const laph_eps_opts* opts, double* theta, double* V,
+                   laph_eps_stats* stats);
+
+/* ------------------------------------------------------------------------
+ * Post-processing, host arrays (recover_spec.cpp, eigensystem.cpp,
+ * read_write.cpp). */
+/* recover_spectrum (recover_spec.cpp:45-54). */
+int laph_recover_spectrum(int nev, const double* theta, int deg, double lambda_l,
+                          double lambda_c, double* evals);
+/* fix_phase (eigensystem.cpp:4-30): V n x nev complex column-major. */
+int laph_fix_phase(int n, int nev, const double* V, double* V_fix, double* phase);
+/* reset_phase (eigensystem.cpp:32-57). */
+int laph_reset_phase(int n, int nev, const double* V, const double* phase_want,
+                     double* V_reset);
+/* check_trace (read_write.cpp:20-35): returns LAPH_OK or LAPH_ERR_TRACE;
+ * trace/sum (complex) may be NULL. */
+int laph_check_trace(int n, int nev, const double* V, double* trace, double* sum);
+/* write_eig_sys_bin (read_write.cpp:176-201): "<dir>/<prefix>.%04d.%03d",
+ * guarded by check_trace. */
+int laph_write_eig_sys_bin(const char* dir, const char* prefix, int config, int t, int n,
+                           int nev, const double* V);
+/* write_eigenvalues_bin (read_write.cpp:237-250). */
+int laph_write_eigenvalues_bin(const char* dir, const char* prefix, int config, int t,
+                               int nev, const double* values);
+
+/* ------------------------------------------------------------------------
+ * Gauge-field ingest (io.cpp:22-146) and the seeded synthetic field. */
+/* read_lime_gauge_field_doubleprec_timeslices: keeps t in [ti, tf],
+ * output (tf-ti+1)*72*V3 doubles in the io.cpp config layout. */
+int laph_read_lime(const char* path, int T, int L, int ti, int tf, double* config);
+/* ILDG LIME writer for a config in the same layout (T*72*V3 doubles). */
+int laph_write_lime(const char* path, int T, int L, const double* config);
+/* U = exp(i eps H), H = sum_a xi_a lambda_a / 2 with xi_a ~ N(0,1) from a
+ * counter-based hash of (seed, t, x, y, z, mu); T*72*V3 doubles. */
+int laph_synth_config(unsigned long long seed, double eps, int L, int T, double* config);
+
+/* ------------------------------------------------------------------------
+ * Mixed-precision batched solver: all timeslices of a configuration on one
+ * device, FP32 Chebyshev filter (correction form), FP64 CholQR2,
+ * Rayleigh-Ritz and residual locking.  Replaces the per-rank timeslice loop
+ * of ev_ts.cpp:176-336 (which spreads timeslices over MPI ranks,
+ * ev_ts.cpp:96-126) for the EPS solve + fix_phase + recover_spectrum part. */
+typedef struct laph_batch laph_batch;
+
+typedef struct laph_batch_opts {
+  int nts;           /* timeslices held by the batch */
+  int nev;           /* eigenpairs per timeslice */
+  int nsub;          /* search-space width (0 = nev + max(16, nev/4)), rounded up to a multiple of 64 */
+  double tol;        /* |D v - lambda v| / ||D|| bound, ||D|| <= 12 */
+  int max_iter;      /* filter / Rayleigh-Ritz cycles (0 = 60) */
+  unsigned long long seed;
+  int warm_start;    /* 1: start from the block set by laph_batch_set_start */
+} laph_batch_opts;
+
+typedef struct laph_batch_stats {
+  int iterations;          /* filter + RR cycles of the slowest timeslice */
+  long long stencil_vecs;  /* FP32 stencil applications (vector count) */
+  long long stencil64_vecs;/* FP64 stencil applications (vector count) */
+  int converged;           /* timeslices with all nev pairs locked */
+  double max_residual;     /* largest |D v - lambda v| over all locked pairs */
+  double seconds;
+  double seconds_filter;   /* device time in the FP32 filter kernels */
+  double seconds_dense;    /* device time in Gram / CholQR / RR / rotations */
+} laph_batch_stats;
+
+int laph_batch_create(laph_batch** out, const laph_params* params, const laph_batch_opts* opts);
+int laph_batch_destroy(laph_batch* b);
+/* Upload timeslice ts (io.cpp layout, 72*V3 doubles) and HYP-smear it on
+ * the device with (alpha_1, alpha_2, iter). */
+int laph_batch_set_timeslice(laph_batch* b, int ts, const double* ts_config, double alpha_1,
+                             double alpha_2, int iter);
+/* Upload already-smeared links of timeslice ts (host link layout). */
+int laph_batch_set_links(laph_batch* b, int ts, const double* links);
+/* Solve every timeslice of the batch. */
+int laph_batch_solve(laph_batch* b, laph_batch_stats* stats);
+/* Results of timeslice ts: eigenvalues of D (descending, i.e. nearest 0
+ * first, like recover_spectrum's output), phase-fixed eigenvectors (n x nev
+ * complex column-major, may be NULL), phases (may be NULL), per-pair
+ * residuals |D v - lambda v| (may be NULL). */
+int laph_batch_get(laph_batch* b, int ts, double* evals, double* V_fix, double* phase,
+                   double* residuals);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,303 @@
+// Blocked multi-vector kernels of the batched solver.  A block of vectors of
+// one timeslice is stored site-major with the vectors fastest:
+//     X[ts][(s*3 + colour)*K + col]            (K a multiple of 64)
+// so one 64-lane wavefront holds 64 vectors of ONE site: the links of that
+// site are wave-uniform and are fetched once per wavefront through scalar
+// loads (the link matrix is read once per 64 vectors), while every vector
+// access is a fully coalesced 512-byte (FP32) / 1 KiB (FP64) row segment.
+// Links per timeslice are AoS [s][dir][9] so a site's 3 forward matrices are
+// one contiguous 216 B (FP32) / 432 B (FP64) scalar fetch.
+//
+// Work-group = 4 wavefronts = 4 consecutive sites of one 64-vector block.
+// Workgroups are handed to the 8 XCDs round-robin; the site-block index is
+// remapped so each XCD sweeps one contiguous slab of sites, which keeps the
+// x-neighbour plane (L*L sites away) in that XCD's L2.
+#include <hip/hip_runtime.h>
+
+#include "kernels_block.h"
+
+namespace laph {
+
+__device__ __forceinline__ int xcd_remap(int bid, int nb) {
+  if ((nb & 7) != 0) return bid;
+  return (bid & 7) * (nb >> 3) + (bid >> 3);
+}
+
+template <class C>
+struct cplx_ops;
+
+template <>
+struct cplx_ops<float2> {
+  using real = float;
+};
+template <>
+struct cplx_ops<double2> {
+  using real = double;
+};
+
+template <class C>
+__device__ __forceinline__ C cfma(C u, C v, C acc) {  // acc += u*v
+  acc.x = fma(u.x, v.x, fma(-u.y, v.y, acc.x));
+  acc.y = fma(u.x, v.y, fma(u.y, v.x, acc.y));
+  return acc;
+}
+template <class C>
+__device__ __forceinline__ C cfmac(C u, C v, C acc) {  // acc += conj(u)*v
+  acc.x = fma(u.x, v.x, fma(u.y, v.y, acc.x));
+  acc.y = fma(u.x, v.y, fma(-u.y, v.x, acc.y));
+  return acc;
+}
+
+// Hopping term h(s) = sum_mu [U_mu(s) v(s+mu) + U_mu(s-mu)^H v(s-mu)] for the
+// lane's vector; returns h and v(s) in acc/ctr.
+template <class C>
+__device__ __forceinline__ void hop(const C* __restrict__ U, const C* __restrict__ v, int s, int L,
+                                    int K, int lane, C acc[3], C ctr[3]) {
+  const int L2 = L * L;
+  const int x = s / L2, y = (s / L) % L, z = s - (s / L) * L;
+  const int up[3] = {x == L - 1 ? s - (L - 1) * L2 : s + L2, y == L - 1 ? s - (L - 1) * L : s + L,
+                     z == L - 1 ? s - (L - 1) : s + 1};
+  const int dn[3] = {x == 0 ? s + (L - 1) * L2 : s - L2, y == 0 ? s + (L - 1) * L : s - L,
+                     z == 0 ? s + (L - 1) : s - 1};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    acc[c] = C{0, 0};
+    ctr[c] = v[(size_t)(3 * s + c) * K + lane];
+  }
+#pragma unroll
+  for (int mu = 0; mu < 3; ++mu) {
+    C xu[3], xm[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      xu[c] = v[(size_t)(3 * up[mu] + c) * K + lane];
+      xm[c] = v[(size_t)(3 * dn[mu] + c) * K + lane];
+    }
+    const C* Uf = U + ((size_t)s * 3 + mu) * 9;       // wave-uniform
+    const C* Ub = U + ((size_t)dn[mu] * 3 + mu) * 9;  // wave-uniform
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        acc[a] = cfma(Uf[3 * a + b], xu[b], acc[a]);
+        acc[a] = cfmac(Ub[3 * b + a], xm[b], acc[a]);
+      }
+    }
+  }
+}
+
+// One correction-form Chebyshev step in FP32 for all timeslices:
+//   out = a1 * (B in) + a2 * r + b * old,   B = c_t (H - 6) + a_t
+// with per-column coefficients (a1, a2, b) for this step.
+__global__ __launch_bounds__(256) void k_filter_step32(const float2* __restrict__ U32,
+                                                        const float2* __restrict__ in,
+                                                        const float2* __restrict__ r,
+                                                        const float2* __restrict__ old,
+                                                        float2* __restrict__ out,
+                                                        const float4* __restrict__ coef,
+                                                        const float2* __restrict__ cab,
+                                                        const int* __restrict__ first_blk, int L,
+                                                        int K, int use_old) {
+  const int ts = blockIdx.z;
+  const int cb = blockIdx.y;
+  if (cb < first_blk[ts]) return;
+  const int V3 = L * L * L;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int s = xcd_remap(blockIdx.x, gridDim.x) * 4 + wave;
+  if (s >= V3) return;
+  const int lane = cb * 64 + (threadIdx.x & 63);
+  const size_t vo = (size_t)ts * V3 * 3 * K;
+  const float2* Ut = U32 + (size_t)ts * V3 * 27;
+  float2 acc[3], ctr[3];
+  hop<float2>(Ut, in + vo, s, L, K, lane, acc, ctr);
+  const float4 cf = coef[(size_t)ts * K + lane];
+  const float2 ca = cab[ts];  // (c_t, a_t - 6 c_t)
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t idx = vo + (size_t)(3 * s + c) * K + lane;
+    float2 bv;
+    bv.x = ca.x * acc[c].x + ca.y * ctr[c].x;
+    bv.y = ca.x * acc[c].y + ca.y * ctr[c].y;
+    const float2 rv = r[idx];
+    float2 o;
+    o.x = cf.x * bv.x + cf.y * rv.x;
+    o.y = cf.x * bv.y + cf.y * rv.y;
+    if (use_old) {
+      const float2 ov = old[idx];
+      o.x += cf.z * ov.x;
+      o.y += cf.z * ov.y;
+    }
+    out[idx] = o;
+  }
+}
+
+// FP64 Laplacian on a block: out = H v - 6 v (= D v) for all columns.
+__global__ __launch_bounds__(256) void k_lap_block64(const double2* __restrict__ U64,
+                                                      const double2* __restrict__ v,
+                                                      double2* __restrict__ out, int L, int K,
+                                                      const int* __restrict__ first_blk) {
+  const int ts = blockIdx.z;
+  const int cb = blockIdx.y;
+  if (first_blk && cb < first_blk[ts]) return;
+  const int V3 = L * L * L;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int s = xcd_remap(blockIdx.x, gridDim.x) * 4 + wave;
+  if (s >= V3) return;
+  const int lane = cb * 64 + (threadIdx.x & 63);
+  const size_t vo = (size_t)ts * V3 * 3 * K;
+  double2 acc[3], ctr[3];
+  hop<double2>(U64 + (size_t)ts * V3 * 27, v + vo, s, L, K, lane, acc, ctr);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double2 o;
+    o.x = acc[c].x - 6.0 * ctr[c].x;
+    o.y = acc[c].y - 6.0 * ctr[c].y;
+    out[vo + (size_t)(3 * s + c) * K + lane] = o;
+  }
+}
+
+// Y = X + F (FP64 + FP32 correction) on the active columns.
+__global__ void k_combine(double2* __restrict__ X, const float2* __restrict__ F, size_t per_ts, int K,
+                          const int* __restrict__ first_blk) {
+  const int ts = blockIdx.y;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= per_ts) return;
+  const int col = (int)(i % K);
+  if (col < first_blk[ts] * 64) return;
+  const size_t idx = (size_t)ts * per_ts + i;
+  const float2 f = F[idx];
+  double2 x = X[idx];
+  x.x += (double)f.x;
+  x.y += (double)f.y;
+  X[idx] = x;
+}
+
+// R = AX - X diag(lam); r32 = c_t * R (B-space residual, FP32); nrm2[ts][col] += |R|^2.
+// grid (row chunks, K/64 column blocks, ts); each thread sums ROWS rows of one column.
+__global__ __launch_bounds__(256) void k_residual(const double2* __restrict__ X,
+                                                   const double2* __restrict__ AX,
+                                                   const double* __restrict__ lam,
+                                                   const double* __restrict__ cts,
+                                                   float2* __restrict__ r32,
+                                                   double* __restrict__ nrm2, int n, int K,
+                                                   int rows_per_block) {
+  const int ts = blockIdx.z;
+  const int col = blockIdx.y * 64 + (threadIdx.x & 63);
+  const int r0 = blockIdx.x * rows_per_block + (threadIdx.x >> 6);
+  const int r1 = min(n, (int)(blockIdx.x + 1) * rows_per_block);
+  const double l = lam[(size_t)ts * K + col];
+  const double c = cts[ts];
+  const size_t off = (size_t)ts * n * K;
+  double acc = 0.0;
+  for (int row = r0; row < r1; row += 4) {
+    const size_t idx = off + (size_t)row * K + col;
+    const double2 a = AX[idx], x = X[idx];
+    const double rx = a.x - l * x.x, ry = a.y - l * x.y;
+    acc += rx * rx + ry * ry;
+    r32[idx] = make_float2((float)(c * rx), (float)(c * ry));
+  }
+  __shared__ double part[256];
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    double s = part[threadIdx.x] + part[threadIdx.x + 64] + part[threadIdx.x + 128] + part[threadIdx.x + 192];
+    atomicAdd(&nrm2[(size_t)ts * K + col], s);
+  }
+}
+
+__global__ void k_fill_zero_d(double* __restrict__ p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+// Output: V[col][row] (column-major n x nev) = e^{-i phi_col} X[row][col],
+// phi_col = arg X[0][col] (fix_phase, eigensystem.cpp:4-30).
+__global__ void k_extract_phase(const double2* __restrict__ X, double2* __restrict__ V,
+                                double* __restrict__ phase, int n, int K, int nev) {
+  const int col = blockIdx.y;
+  const double2 x0 = X[col];
+  const double ph = atan2(x0.y, x0.x);
+  const double cr = cos(ph), si = -sin(ph);
+  if (blockIdx.x == 0 && threadIdx.x == 0) phase[col] = ph;
+  for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < n; row += gridDim.x * blockDim.x) {
+    const double2 x = X[(size_t)row * K + col];
+    V[(size_t)col * n + row] = make_double2(x.x * cr - x.y * si, x.x * si + x.y * cr);
+  }
+}
+
+__global__ void k_convert_links32(const double2* __restrict__ U, float2* __restrict__ U32, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) U32[i] = make_float2((float)U[i].x, (float)U[i].y);
+}
+
+// Seeded Gaussian start block (counter hash per element), FP64.
+__global__ void k_random_block(double2* __restrict__ X, size_t total, unsigned long long seed) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  auto mix = [](unsigned long long z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  unsigned long long h1 = mix(seed * 0x9E3779B97F4A7C15ull + (2 * i + 1) * 0xD1B54A32D192ED03ull);
+  unsigned long long h2 = mix(seed * 0x9E3779B97F4A7C15ull + (2 * i + 2) * 0xD1B54A32D192ED03ull);
+  double u1 = ((double)(h1 >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  double u2 = ((double)(h2 >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  double rr = sqrt(-2.0 * log(u1));
+  X[i] = make_double2(rr * cos(2.0 * M_PI * u2), rr * sin(2.0 * M_PI * u2));
+}
+
+// ---------------------------------------------------------------------------
+static inline dim3 site_grid(int L, int K, int nts) {
+  const int V3 = L * L * L;
+  return dim3((V3 + 3) / 4, K / 64, nts);
+}
+
+hipError_t launch_filter_step32(const float2* U32, const float2* in, const float2* r, const float2* old,
+                                float2* out, const float4* coef, const float2* cab, const int* first_blk,
+                                int L, int K, int nts, hipStream_t st) {
+  k_filter_step32<<<site_grid(L, K, nts), 256, 0, st>>>(U32, in, r, old, out, coef, cab, first_blk, L, K,
+                                                        old != nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_lap_block64(const double2* U64, const double2* v, double2* out, int L, int K, int nts,
+                              const int* first_blk, hipStream_t st) {
+  k_lap_block64<<<site_grid(L, K, nts), 256, 0, st>>>(U64, v, out, L, K, first_blk);
+  return hipGetLastError();
+}
+
+hipError_t launch_combine(double2* X, const float2* F, size_t per_ts, int K, int nts, const int* first_blk,
+                          hipStream_t st) {
+  dim3 g((unsigned)((per_ts + 255) / 256), nts);
+  k_combine<<<g, 256, 0, st>>>(X, F, per_ts, K, first_blk);
+  return hipGetLastError();
+}
+
+hipError_t launch_residual(const double2* X, const double2* AX, const double* lam, const double* cts,
+                           float2* r32, double* nrm2, int n, int K, int nts, hipStream_t st) {
+  const size_t nn = (size_t)nts * K;
+  k_fill_zero_d<<<(unsigned)((nn + 255) / 256), 256, 0, st>>>(nrm2, nn);
+  const int rows = 256;
+  dim3 g((n + rows - 1) / rows, K / 64, nts);
+  k_residual<<<g, 256, 0, st>>>(X, AX, lam, cts, r32, nrm2, n, K, rows);
+  return hipGetLastError();
+}
+
+hipError_t launch_extract_phase(const double2* X, double2* V, double* phase, int n, int K, int nev,
+                                hipStream_t st) {
+  dim3 g(64, nev);
+  k_extract_phase<<<g, 256, 0, st>>>(X, V, phase, n, K, nev);
+  return hipGetLastError();
+}
+
+hipError_t launch_convert_links32(const double2* U, float2* U32, size_t n, hipStream_t st) {
+  k_convert_links32<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(U, U32, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_random_block(double2* X, size_t total, unsigned long long seed, hipStream_t st) {
+  k_random_block<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(X, total, seed);
+  return hipGetLastError();
+}
+
+}  // namespace laph
