@@ -158,7 +158,7 @@ typedef struct laph_batch_opts {
   int nts;           /* timeslices held by the batch */
   int nev;           /* eigenpairs per timeslice */
   int nsub;          /* search-space width (0 = nev + max(16, nev/4)), rounded up to a multiple of 64 */
-  double tol;        /* |D v - lambda v| / ||D|| bound, ||D|| <= 12 */
+  double tol;        /* lock when |D v - lambda v| <= tol (v unit norm) */
   int max_iter;      /* filter / Rayleigh-Ritz cycles (0 = 60) */
   unsigned long long seed;
   int warm_start;    /* 1: start from the block set by laph_batch_set_start */
@@ -173,6 +173,9 @@ typedef struct laph_batch_stats {
   double seconds;
   double seconds_filter;   /* device time in the FP32 filter kernels */
   double seconds_dense;    /* device time in Gram / CholQR / RR / rotations */
+  long long filter_launches; /* FP32 filter-step kernel launches */
+  double filter_bytes;     /* algorithmic bytes of those launches (vectors read/written
+                              once, links once per 64-vector block) */
 } laph_batch_stats;
 
 int laph_batch_create(laph_batch** out, const laph_params* params, const laph_batch_opts* opts);

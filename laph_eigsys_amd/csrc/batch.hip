@@ -324,14 +324,26 @@ static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
   std::vector<double> lc(nts);
   int active = 0;
   LAPH_TRY(residuals_and_window(b, lc, &active));
-  long long sv32 = 0, sv64 = (long long)nts * K;
+  long long sv32 = 0, sv64 = (long long)nts * K, flaunch = 0;
+  double fbytes = 0;
   float ms_filter = 0, ms_dense = 0;
   const int max_iter = b->o.max_iter > 0 ? b->o.max_iter : 60;
   int it = 0;
   for (it = 0; it < max_iter && active > 0; ++it) {
     LAPH_TRY(upload_coefficients(b, lc));
-    for (int t = 0; t < nts; ++t)
-      if (b->nlock[t] < b->nev) sv32 += (long long)(b->deg - 1) * (K - (b->nlock[t] / 64) * 64);
+    for (int t = 0; t < nts; ++t) {
+      if (b->nlock[t] >= b->nev) continue;
+      const int cols = K - (b->nlock[t] / 64) * 64;
+      sv32 += (long long)(b->deg - 1) * cols;
+      // algorithmic bytes: step 1 reads r and writes f_2 (2 vector touches),
+      // step 2 adds r (3), later steps add f_{k-1} (4); a vector touch is
+      // 3 colours x 8 B per site; links 27 x 8 B per site per 64-vector block
+      for (int s = 1; s < b->deg; ++s) {
+        const int touches = s == 1 ? 2 : (s == 2 ? 3 : 4);
+        fbytes += (double)b->V3 * ((double)cols * touches * 24.0 + (cols / 64) * 216.0);
+      }
+    }
+    flaunch += b->deg - 1;
     float2* fm = nullptr;
     LAPH_HIP(hipEventRecord(b->ev[0], st));
     LAPH_TRY(run_filter(b, &fm));
@@ -368,6 +380,8 @@ static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
     stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     stats->seconds_filter = ms_filter * 1e-3;
     stats->seconds_dense = ms_dense * 1e-3;
+    stats->filter_launches = flaunch;
+    stats->filter_bytes = fbytes;
   }
   if (conv < nts)
     return set_error(LAPH_ERR_NOCONV, "laph_batch_solve: %d of %d timeslices converged after %d cycles", conv, nts,
