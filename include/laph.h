@@ -188,12 +188,51 @@ int laph_batch_set_timeslice(laph_batch* b, int ts, const double* ts_config, dou
 int laph_batch_set_links(laph_batch* b, int ts, const double* links);
 /* Solve every timeslice of the batch. */
 int laph_batch_solve(laph_batch* b, laph_batch_stats* stats);
+/* Device time of the FP32 filter-step kernel, alone on the device: replays
+ * the captured filter of the first half of the timeslices (every column
+ * active) reps times, timed with HIP events on its stream.  Outputs the
+ * average launch duration and the algorithmic bytes per launch (vectors read
+ * or written once per step, links once per 64-vector block).  Requires a
+ * solved batch (its coefficient tables).  No counterpart in the reference. */
+int laph_batch_time_filter(laph_batch* b, int reps, double* ms_per_launch, double* bytes_per_launch);
 /* Results of timeslice ts: eigenvalues of D (descending, i.e. nearest 0
  * first, like recover_spectrum's output), phase-fixed eigenvectors (n x nev
  * complex column-major, may be NULL), phases (may be NULL), per-pair
  * residuals |D v - lambda v| (may be NULL). */
 int laph_batch_get(laph_batch* b, int ts, double* evals, double* V_fix, double* phase,
                    double* residuals);
+
+/* ------------------------------------------------------------------------
+ * Learned warm start: a gauge-covariant parallel-transport network (16
+ * complex channels per group of 16 output vectors, BF16 activations, FP32
+ * arithmetic) that maps Gaussian colour vectors to the starting block of the
+ * batched solver.  No counterpart in the reference (its EPS starts from
+ * SLEPc's random vector); the random start stays the fallback: a batch
+ * solved after laph_net_propose is judged by the same residual test. */
+typedef struct laph_net laph_net;
+
+int laph_net_create(laph_net** out, int layers, unsigned long long seed, int device);
+int laph_net_destroy(laph_net* net);
+/* Number of float parameters: layers*7*16*16 complex weights, then
+ * layers*16 gate biases. */
+int laph_net_num_params(laph_net* net);
+int laph_net_get_params(laph_net* net, float* params);
+int laph_net_set_params(laph_net* net, const float* params);
+/* Writes the network's output for fresh Gaussian inputs into the start block
+ * of every timeslice of b (K columns each); laph_batch_solve uses it when
+ * the batch was created with warm_start = 1. */
+int laph_net_propose(laph_net* net, laph_batch* b, unsigned long long seed);
+/* One Adam step (learning rate lr) on the converged eigenvectors of the last
+ * laph_batch_solve of b; loss = -(mean fraction of each output vector inside
+ * the span of the nev eigenvectors). */
+int laph_net_train_step(laph_net* net, laph_batch* b, unsigned long long seed, double lr, double* loss);
+/* Loss, output and parameter gradient for one timeslice given on the host:
+ * links (host link layout), input (n x K complex FP32, [3*s+colour][K]),
+ * targets V (n x nev complex FP64, [3*s+colour][nev]); output (n x K complex
+ * FP64, may be NULL), grad (num_params floats, real gradients of Re W, Im W
+ * and b, may be NULL).  fp32_act = 1 stores activations in FP32. */
+int laph_net_loss_grad(laph_net* net, int L, int K, int fp32_act, const double* links, const float* input,
+                       const double* V, int nev, double* loss, double* output, float* grad);
 
 #ifdef __cplusplus
 }

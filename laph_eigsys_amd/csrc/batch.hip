@@ -15,13 +15,18 @@
 //     f_{k+1} = a1 (B f_k) + a2 r + b f_{k-1}     (per-column a1, a2, b)
 // and the filtered vector into y = (T_m(theta)/s_m) x + f_m.
 //
-// Per cycle (all timeslices batched):
+// The timeslices are split into two lanes, each driven by its own host
+// thread on its own stream, so that one lane's HBM-bound filter overlaps the
+// other lane's MFMA-bound dense steps.  Per cycle (every timeslice of a lane
+// batched):
 //   1. FP32 filter, m-1 fused stencil steps (hipGraph, one launch per step
 //      covering every active vector block of every timeslice)
 //   2. Y = gamma X + F (FP64)
 //   3. CholQR2 in FP64 (Gram zgemm, zpotrf, ztrsm; all strided-batched)
 //   4. AY = D Y (FP64 blocked stencil)
-//   5. Rayleigh-Ritz in FP64: H = Y^H AY, zheevd, X = Y Q, AX = AY Q
+//   5. Rayleigh-Ritz in FP64: H = Y^H AY (and, after the first cycles, the
+//      Gram G = Y^H Y instead of an explicit CholQR2), zheevd, X = Y Q,
+//      AX = D X by the FP64 blocked stencil
 //   6. residuals R = AX - X Lambda in FP64; lock the leading columns with
 //      |R_j| <= tol; window lower bound = largest Ritz value of -D.
 // The window map is B = c_t D + a_t with c_t = 2/(lambda_l - lambda_c,t),
@@ -41,49 +46,9 @@
 #include "kernels.h"
 #include "kernels_block.h"
 
-#define LAPH_RB(call)                                                                               \
-  do {                                                                                              \
-    rocblas_status s_ = (call);                                                                     \
-    if (s_ != rocblas_status_success)                                                               \
-      return laph::set_error(LAPH_ERR_HIP, "%s failed: %s", #call, rocblas_status_to_string(s_));   \
-  } while (0)
+#include "batch_internal.h"
 
-struct laph_batch {
-  laph_params p{};
-  laph_batch_opts o{};
-  int V3 = 0, n = 0, K = 0, nts = 0, nev = 0, deg = 0;
-  size_t per_ts = 0;  // n*K
-  hipStream_t st = nullptr;
-  rocblas_handle rb = nullptr;
-  // links
-  double2* U64 = nullptr;  // [nts][V3*27]
-  float2* U32 = nullptr;
-  double2* dec = nullptr;   // HYP scratch for one timeslice
-  double2* Unew = nullptr;
-  // blocks
-  double2* B[3] = {nullptr, nullptr, nullptr};  // rotating roles X, AX, T
-  int iX = 0, iAX = 1, iT = 2;
-  float2* R32 = nullptr;
-  float2* P[3] = {nullptr, nullptr, nullptr};
-  // small per-timeslice data
-  double2* G = nullptr;      // [nts][K*K]
-  double* mu = nullptr;      // [nts][K]
-  double* E = nullptr;       // [nts][K]
-  rocblas_int* info = nullptr;
-  double* nrm2 = nullptr;    // [nts][K]
-  double* lam = nullptr;     // [nts][K]
-  double* cts = nullptr;     // [nts]
-  double* gam = nullptr;     // [nts][K]
-  float4* coef = nullptr;    // [deg-1][nts][K]
-  float2* cab = nullptr;     // [nts]
-  int* first_blk = nullptr;  // [nts]
-  // state
-  std::vector<int> nlock;
-  std::vector<double> h_mu, h_res;
-  bool solved = false;
-  hipGraphExec_t filter_graph = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-};
+#include <thread>
 
 namespace laph {
 
@@ -97,6 +62,8 @@ static int batch_alloc(laph_batch* b) {
   for (int i = 0; i < 3; ++i) LAPH_HIP(hipMalloc(&b->P[i], nb * sizeof(float2)));
   const size_t kk = (size_t)b->nts * b->K * b->K;
   LAPH_HIP(hipMalloc(&b->G, kk * sizeof(double2)));
+  LAPH_HIP(hipMalloc(&b->G2, kk * sizeof(double2)));
+  LAPH_HIP(hipMalloc(&b->P2, kk * sizeof(double2)));
   const size_t nk = (size_t)b->nts * b->K;
   LAPH_HIP(hipMalloc(&b->mu, nk * sizeof(double)));
   LAPH_HIP(hipMalloc(&b->E, nk * sizeof(double)));
@@ -108,7 +75,20 @@ static int batch_alloc(laph_batch* b) {
   LAPH_HIP(hipMalloc(&b->coef, (size_t)std::max(1, b->deg - 1) * nk * sizeof(float4)));
   LAPH_HIP(hipMalloc(&b->cab, b->nts * sizeof(float2)));
   LAPH_HIP(hipMalloc(&b->first_blk, b->nts * sizeof(int)));
-  for (int i = 0; i < 4; ++i) LAPH_HIP(hipEventCreate(&b->ev[i]));
+  LAPH_HIP(hipMalloc(&b->lock_blk, b->nts * sizeof(int)));
+  LAPH_HIP(hipMalloc(&b->cn, nk * sizeof(double)));
+  const int nlanes = b->nts >= 2 ? 2 : 1;
+  b->lanes.resize(nlanes);
+  for (int h = 0; h < nlanes; ++h) {
+    laph_lane& ln = b->lanes[h];
+    ln.t0 = h * (b->nts / nlanes);
+    ln.nt = (h == nlanes - 1) ? b->nts - ln.t0 : b->nts / nlanes;
+    LAPH_HIP(hipStreamCreateWithFlags(&ln.st, hipStreamNonBlocking));
+    LAPH_CHECK(rocblas_create_handle(&ln.rb) == rocblas_status_success &&
+                   rocblas_set_stream(ln.rb, ln.st) == rocblas_status_success,
+               LAPH_ERR_HIP, "laph_batch_create: rocBLAS handle");
+    for (auto& e : ln.ev) LAPH_HIP(hipEventCreate(&e));
+  }
   return LAPH_OK;
 }
 
@@ -116,126 +96,254 @@ static void batch_free(laph_batch* b) {
   if (!b) return;
   (void)hipSetDevice(b->p.device);
   if (b->st) (void)hipStreamSynchronize(b->st);
-  if (b->filter_graph) (void)hipGraphExecDestroy(b->filter_graph);
-  void* bufs[] = {b->U64, b->U32, b->dec, b->Unew, b->B[0], b->B[1], b->B[2], b->R32, b->P[0], b->P[1],
-                  b->P[2], b->G, b->mu, b->E, b->info, b->nrm2, b->lam, b->cts, b->gam, b->coef, b->cab,
-                  b->first_blk};
+  for (auto& ln : b->lanes) {
+    if (ln.st) (void)hipStreamSynchronize(ln.st);
+    if (ln.graph) (void)hipGraphExecDestroy(ln.graph);
+    for (auto e : ln.ev)
+      if (e) (void)hipEventDestroy(e);
+    if (ln.rb) rocblas_destroy_handle(ln.rb);
+    if (ln.st) (void)hipStreamDestroy(ln.st);
+  }
+  void* bufs[] = {b->U64, b->U32, b->dec,  b->Unew, b->B[0], b->B[1],   b->B[2], b->R32,     b->P[0],
+                  b->P[1], b->P[2], b->G,  b->G2,   b->P2,   b->mu,     b->E,    b->info,    b->nrm2,
+                  b->lam,  b->cts,  b->gam, b->coef, b->cab, b->first_blk, b->lock_blk, b->cn};
   for (void* q : bufs)
     if (q) (void)hipFree(q);
-  for (auto e : b->ev)
-    if (e) (void)hipEventDestroy(e);
-  if (b->rb) rocblas_destroy_handle(b->rb);
   if (b->st) (void)hipStreamDestroy(b->st);
   delete b;
 }
 
 using cplx = rocblas_double_complex;
 static inline cplx* zc(double2* p) { return reinterpret_cast<cplx*>(p); }
+static inline cplx* zc(const double2* p) { return reinterpret_cast<cplx*>(const_cast<double2*>(p)); }
 
-// CholQR on every timeslice: Yc (K x n column-major = row-major n x K)
-// <- U^{-H} Yc with Yc Yc^H = U^H U.
-static int cholqr(laph_batch* b, double2* Y) {
-  const int K = b->K, n = b->n, nts = b->nts;
-  const cplx one(1.0, 0.0), zero(0.0, 0.0);
-  LAPH_RB(rocblas_zgemm_strided_batched(b->rb, rocblas_operation_none, rocblas_operation_conjugate_transpose, K, K,
-                                        n, &one, zc(Y), K, (rocblas_stride)b->per_ts, zc(Y), K,
-                                        (rocblas_stride)b->per_ts, &zero, zc(b->G), K, (rocblas_stride)K * K, nts));
-  LAPH_RB(rocsolver_zpotrf_strided_batched(b->rb, rocblas_fill_upper, K, zc(b->G), K, (rocblas_stride)K * K,
-                                           b->info, nts));
-  std::vector<rocblas_int> info(nts);
-  LAPH_HIP(hipMemcpyAsync(info.data(), b->info, nts * sizeof(rocblas_int), hipMemcpyDeviceToHost, b->st));
-  LAPH_HIP(hipStreamSynchronize(b->st));
-  for (int t = 0; t < nts; ++t)
-    LAPH_CHECK(info[t] == 0, LAPH_ERR_NOCONV, "CholQR: Gram matrix of timeslice %d not positive definite (info %d)",
-               t, (int)info[t]);
-  LAPH_RB(rocblas_ztrsm_strided_batched(b->rb, rocblas_side_left, rocblas_fill_upper,
-                                        rocblas_operation_conjugate_transpose, rocblas_diagonal_non_unit, K, n, &one,
-                                        zc(b->G), K, (rocblas_stride)K * K, zc(Y), K, (rocblas_stride)b->per_ts, nts));
+// The dense steps act on the active block: columns [l, K) of every
+// timeslice of the lane, l = ln.lock (a multiple of 64; columns below l are
+// hard-locked converged eigenvectors).  In the column-major view the block of
+// one timeslice is Yc = K x n with leading dimension K (the site-major
+// [n][K] storage), so the active block is the sub-matrix starting at row l.
+struct Act {
+  int l, a;  // first active column, active width
+};
+
+// Lane-local views of the batch arrays.
+struct View {
+  laph_batch* b;
+  laph_lane* ln;
+  double2* blk(int i) const { return b->B[i] + (size_t)ln->t0 * b->per_ts; }
+  double2* X() const { return blk(ln->iX); }
+  double2* AX() const { return blk(ln->iAX); }
+  double2* T() const { return blk(ln->iT); }
+  size_t kk() const { return (size_t)ln->t0 * b->K * b->K; }
+  size_t nk() const { return (size_t)ln->t0 * b->K; }
+  size_t vo() const { return (size_t)ln->t0 * b->per_ts; }
+  size_t lo() const { return (size_t)ln->t0 * b->V3 * 27; }
+};
+
+static int dense_sync_info(const View& v, const char* what) {
+  std::vector<rocblas_int> info(v.ln->nt);
+  LAPH_HIP(hipMemcpyAsync(info.data(), v.b->info + v.ln->t0, v.ln->nt * sizeof(rocblas_int), hipMemcpyDeviceToHost,
+                          v.ln->st));
+  LAPH_HIP(hipStreamSynchronize(v.ln->st));
+  for (int t = 0; t < v.ln->nt; ++t)
+    if (info[t] != 0)
+      return set_error(LAPH_ERR_NOCONV, "%s failed on timeslice %d (info %d)", what, v.ln->t0 + t, (int)info[t]);
   return LAPH_OK;
 }
 
-// Rayleigh-Ritz: with Y orthonormal in B[iX] and AY in B[iAX]:
-// M = -(AYc Yc^H) = -conj(H); M = W diag(mu) W^H (mu ascending = eigenvalues
-// of -D, nearest 0 first); X <- Y conj(W) i.e. Xc = W^H Yc; same for AX.
-static int rayleigh_ritz(laph_batch* b) {
-  const int K = b->K, n = b->n, nts = b->nts;
+// Gt = alpha * A_c B_c^H (a x a, column-major, lda K) for the active blocks.
+static int gram(const View& v, Act q, const double2* A, const double2* Bm, double2* Gt, double alpha) {
+  const int K = v.b->K;
+  const cplx al(alpha, 0.0), zero(0.0, 0.0);
+  LAPH_RB(rocblas_zgemm_strided_batched(v.ln->rb, rocblas_operation_none, rocblas_operation_conjugate_transpose, q.a,
+                                        q.a, v.b->n, &al, zc(A + q.l), K, (rocblas_stride)v.b->per_ts, zc(Bm + q.l), K,
+                                        (rocblas_stride)v.b->per_ts, &zero, zc(Gt), K, (rocblas_stride)K * K,
+                                        v.ln->nt));
+  return LAPH_OK;
+}
+
+// Y_A <- Y_A - X_L (X_L^H Y_A): one classical Gram-Schmidt pass against the
+// hard-locked eigenvectors (the filtered block is near-orthogonal to them
+// already: the residuals that drive the filter are orthogonal to X_L up to
+// the locked residual).
+static int project_locked(const View& v, Act q) {
+  if (q.l == 0) return LAPH_OK;
+  const int K = v.b->K;
   const cplx one(1.0, 0.0), zero(0.0, 0.0), mone(-1.0, 0.0);
-  double2* Y = b->B[b->iX];
-  double2* AY = b->B[b->iAX];
-  double2* T = b->B[b->iT];
-  LAPH_RB(rocblas_zgemm_strided_batched(b->rb, rocblas_operation_none, rocblas_operation_conjugate_transpose, K, K,
-                                        n, &mone, zc(AY), K, (rocblas_stride)b->per_ts, zc(Y), K,
-                                        (rocblas_stride)b->per_ts, &zero, zc(b->G), K, (rocblas_stride)K * K, nts));
-  LAPH_RB(rocsolver_zheevd_strided_batched(b->rb, rocblas_evect_original, rocblas_fill_upper, K, zc(b->G), K,
-                                           (rocblas_stride)K * K, b->mu, K, b->E, K, b->info, nts));
-  // T <- W^H Y  (new X), then Y's buffer <- W^H AY (new AX)
-  LAPH_RB(rocblas_zgemm_strided_batched(b->rb, rocblas_operation_conjugate_transpose, rocblas_operation_none, K, n,
-                                        K, &one, zc(b->G), K, (rocblas_stride)K * K, zc(Y), K,
-                                        (rocblas_stride)b->per_ts, &zero, zc(T), K, (rocblas_stride)b->per_ts, nts));
-  LAPH_RB(rocblas_zgemm_strided_batched(b->rb, rocblas_operation_conjugate_transpose, rocblas_operation_none, K, n,
-                                        K, &one, zc(b->G), K, (rocblas_stride)K * K, zc(AY), K,
-                                        (rocblas_stride)b->per_ts, &zero, zc(Y), K, (rocblas_stride)b->per_ts, nts));
-  // roles: X <- T, AX <- old Y buffer, T <- old AY buffer
-  const int oX = b->iX, oAX = b->iAX, oT = b->iT;
-  b->iX = oT;
-  b->iAX = oX;
-  b->iT = oAX;
-  std::vector<rocblas_int> info(nts);
-  LAPH_HIP(hipMemcpyAsync(info.data(), b->info, nts * sizeof(rocblas_int), hipMemcpyDeviceToHost, b->st));
-  LAPH_HIP(hipMemcpyAsync(b->h_mu.data(), b->mu, b->h_mu.size() * sizeof(double), hipMemcpyDeviceToHost, b->st));
-  LAPH_HIP(hipStreamSynchronize(b->st));
-  for (int t = 0; t < nts; ++t)
-    LAPH_CHECK(info[t] == 0, LAPH_ERR_NOCONV, "Rayleigh-Ritz: zheevd failed on timeslice %d (info %d)", t,
-               (int)info[t]);
+  double2* X = v.X();
+  double2* P2 = v.b->P2 + v.kk();
+  LAPH_RB(rocblas_zgemm_strided_batched(v.ln->rb, rocblas_operation_none, rocblas_operation_conjugate_transpose, q.a,
+                                        q.l, v.b->n, &one, zc(X + q.l), K, (rocblas_stride)v.b->per_ts, zc(X), K,
+                                        (rocblas_stride)v.b->per_ts, &zero, zc(P2), K, (rocblas_stride)K * K,
+                                        v.ln->nt));
+  LAPH_RB(rocblas_zgemm_strided_batched(v.ln->rb, rocblas_operation_none, rocblas_operation_none, q.a, v.b->n, q.l,
+                                        &mone, zc(P2), K, (rocblas_stride)K * K, zc(X), K,
+                                        (rocblas_stride)v.b->per_ts, &one, zc(X + q.l), K,
+                                        (rocblas_stride)v.b->per_ts, v.ln->nt));
   return LAPH_OK;
 }
 
-// Window per timeslice from the Ritz values, residuals, lock counts.
-// Returns the number of timeslices still unconverged.
-static int residuals_and_window(laph_batch* b, std::vector<double>& lc, int* n_active) {
-  const int K = b->K, nts = b->nts;
+// Explicit CholQR of the active block: Yc_A <- U^{-H} Yc_A, Yc_A Yc_A^H = U^H U.
+static int cholqr(const View& v, Act q) {
+  const int K = v.b->K;
+  const cplx one(1.0, 0.0);
+  double2* Y = v.X();
+  double2* G = v.b->G + v.kk();
+  // unit columns first: the filtered columns of the first cycles differ in
+  // norm by many orders of magnitude (the span is unchanged by the scaling)
+  LAPH_HIP(launch_normalize_columns(Y, v.b->cn + v.nk(), v.b->n, K, v.ln->nt, q.l, v.ln->st));
+  LAPH_TRY(gram(v, q, Y, Y, G, 1.0));
+  LAPH_RB(rocsolver_zpotrf_strided_batched(v.ln->rb, rocblas_fill_upper, q.a, zc(G), K, (rocblas_stride)K * K,
+                                           v.b->info + v.ln->t0, v.ln->nt));
+  LAPH_TRY(dense_sync_info(v, "CholQR (zpotrf)"));
+  LAPH_RB(rocblas_ztrsm_strided_batched(v.ln->rb, rocblas_side_left, rocblas_fill_upper,
+                                        rocblas_operation_conjugate_transpose, rocblas_diagonal_non_unit, q.a, v.b->n,
+                                        &one, zc(G), K, (rocblas_stride)K * K, zc(Y + q.l), K,
+                                        (rocblas_stride)v.b->per_ts, v.ln->nt));
+  return LAPH_OK;
+}
+
+// Rayleigh-Ritz of the active block Y_A (in X) with AY_A = D Y_A (in AX).
+// With Mt = -(AYc_A Yc_A^H) = -conj(H) and, when the block is not
+// orthonormalised explicitly, Gt = Yc_A Yc_A^H = conj(G) = U^H U, the
+// generalized problem Mt z = mu Gt z becomes the standard Hermitian problem
+// U^{-H} Mt U^{-1} w = mu w, z = U^{-1} w.  mu ascending = eigenvalues of -D
+// nearest 0 first.  New block: Xc_A = Z^H Yc_A; AX_A = D X_A by the FP64
+// blocked stencil (bandwidth-bound, far cheaper than rotating AY_A by Z).
+static int rayleigh_ritz(const View& v, Act q, bool orthonormal, bool* chol_failed) {
+  laph_batch* b = v.b;
+  const int K = b->K, nt = v.ln->nt;
+  const cplx one(1.0, 0.0), zero(0.0, 0.0);
+  double2* Y = v.X();
+  double2* AY = v.AX();
+  double2* T = v.T();
+  double2* G = b->G + v.kk();
+  double2* G2 = b->G2 + v.kk();
+  rocblas_int* info = b->info + v.ln->t0;
+  const rocblas_stride kk = (rocblas_stride)K * K;
+  rocblas_handle rb = v.ln->rb;
+  *chol_failed = false;
+  LAPH_TRY(gram(v, q, AY, Y, G, -1.0));  // Mt
+  if (!orthonormal) {
+    LAPH_TRY(gram(v, q, Y, Y, G2, 1.0));  // Gt
+    LAPH_RB(rocsolver_zpotrf_strided_batched(rb, rocblas_fill_upper, q.a, zc(G2), K, kk, info, nt));
+    if (dense_sync_info(v, "Rayleigh-Ritz (zpotrf)") != LAPH_OK) {
+      *chol_failed = true;
+      return LAPH_OK;
+    }
+    LAPH_RB(rocblas_ztrsm_strided_batched(rb, rocblas_side_left, rocblas_fill_upper,
+                                          rocblas_operation_conjugate_transpose, rocblas_diagonal_non_unit, q.a, q.a,
+                                          &one, zc(G2), K, kk, zc(G), K, kk, nt));
+    LAPH_RB(rocblas_ztrsm_strided_batched(rb, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                                          rocblas_diagonal_non_unit, q.a, q.a, &one, zc(G2), K, kk, zc(G), K, kk, nt));
+  }
+  LAPH_RB(rocsolver_zheevd_strided_batched(rb, rocblas_evect_original, rocblas_fill_upper, q.a, zc(G), K, kk,
+                                           b->mu + v.nk() + q.l, K, b->E + v.nk(), K, info, nt));
+  LAPH_TRY(dense_sync_info(v, "Rayleigh-Ritz (zheevd)"));
+  if (!orthonormal)
+    LAPH_RB(rocblas_ztrsm_strided_batched(rb, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                                          rocblas_diagonal_non_unit, q.a, q.a, &one, zc(G2), K, kk, zc(G), K, kk, nt));
+  LAPH_RB(rocblas_zgemm_strided_batched(rb, rocblas_operation_conjugate_transpose, rocblas_operation_none, q.a, b->n,
+                                        q.a, &one, zc(G), K, kk, zc(Y + q.l), K, (rocblas_stride)b->per_ts, &zero,
+                                        zc(T + q.l), K, (rocblas_stride)b->per_ts, nt));
+  LAPH_HIP(launch_lap_block64(b->U64 + v.lo(), T, AY, b->p.L, K, nt, b->lock_blk + v.ln->t0, v.ln->st));
+  // the hard-locked columns of X live in the old X buffer: carry them over
+  if (q.l > 0)
+    LAPH_HIP(hipMemcpy2DAsync(T, (size_t)K * sizeof(double2), Y, (size_t)K * sizeof(double2),
+                              (size_t)q.l * sizeof(double2), (size_t)nt * b->n, hipMemcpyDeviceToDevice, v.ln->st));
+  // roles: X <- T, AX <- AY buffer (now D X), T <- old X buffer
+  std::swap(v.ln->iX, v.ln->iT);
+  LAPH_HIP(hipMemcpyAsync(b->h_mu.data() + v.nk(), b->mu + v.nk(), (size_t)nt * K * sizeof(double),
+                          hipMemcpyDeviceToHost, v.ln->st));
+  LAPH_HIP(hipStreamSynchronize(v.ln->st));
+  return LAPH_OK;
+}
+
+// Orthonormalise (explicitly when asked or when the Gram Cholesky fails),
+// apply D and run Rayleigh-Ritz on the active block.
+static int orthonormalise_and_rr(const View& v, Act q, bool explicit_qr) {
+  laph_batch* b = v.b;
+  for (int pass = 0; pass < 2; ++pass) {
+    if (explicit_qr || pass == 1) {
+      LAPH_TRY(cholqr(v, q));
+      LAPH_TRY(cholqr(v, q));
+    }
+    LAPH_HIP(launch_lap_block64(b->U64 + v.lo(), v.X(), v.AX(), b->p.L, b->K, v.ln->nt, b->lock_blk + v.ln->t0,
+                                v.ln->st));
+    v.ln->sv64 += 2LL * v.ln->nt * q.a;  // D Y for Rayleigh-Ritz, D X after it
+    bool failed = false;
+    LAPH_TRY(rayleigh_ritz(v, q, explicit_qr || pass == 1, &failed));
+    if (!failed) return LAPH_OK;
+  }
+  return set_error(LAPH_ERR_NOCONV, "Rayleigh-Ritz: Gram matrix not positive definite after CholQR2");
+}
+
+// Window per timeslice from the Ritz values, residuals of the active block,
+// lock counts.  Sets *n_active to the number of unconverged timeslices.
+static int residuals_and_window(const View& v, std::vector<double>& lc, int* n_active) {
+  laph_batch* b = v.b;
+  laph_lane* ln = v.ln;
+  const int K = b->K, nt = ln->nt;
   const double lamL = b->p.lambda_l;
-  std::vector<double> hl((size_t)nts * K), hc(nts);
-  for (int t = 0; t < nts; ++t) {
+  std::vector<double> hl((size_t)nt * K), hc(nt);
+  for (int t = 0; t < nt; ++t) {
+    const double* mu = &b->h_mu[v.nk() + (size_t)t * K];
     // lower end of the damped interval: largest Ritz value of -D, kept
     // well below lambda_l so the amplification stays inside FP32 range
-    lc[t] = std::min(b->h_mu[(size_t)t * K + K - 1], 0.75 * lamL);
+    lc[t] = std::min(mu[K - 1], 0.75 * lamL);
     hc[t] = 2.0 / (lamL - lc[t]);
-    for (int j = 0; j < K; ++j) hl[(size_t)t * K + j] = -b->h_mu[(size_t)t * K + j];
+    for (int j = 0; j < K; ++j) hl[(size_t)t * K + j] = -mu[j];
   }
-  LAPH_HIP(hipMemcpyAsync(b->lam, hl.data(), hl.size() * sizeof(double), hipMemcpyHostToDevice, b->st));
-  LAPH_HIP(hipMemcpyAsync(b->cts, hc.data(), hc.size() * sizeof(double), hipMemcpyHostToDevice, b->st));
-  LAPH_HIP(launch_residual(b->B[b->iX], b->B[b->iAX], b->lam, b->cts, b->R32, b->nrm2, b->n, K, nts, b->st));
-  LAPH_HIP(hipMemcpyAsync(b->h_res.data(), b->nrm2, b->h_res.size() * sizeof(double), hipMemcpyDeviceToHost, b->st));
-  LAPH_HIP(hipStreamSynchronize(b->st));
-  int active = 0;
-  for (int t = 0; t < nts; ++t) {
-    int nl = 0;
-    while (nl < K && std::sqrt(b->h_res[(size_t)t * K + nl]) <= b->o.tol) ++nl;
-    b->nlock[t] = nl;
+  LAPH_HIP(hipMemcpyAsync(b->lam + v.nk(), hl.data(), hl.size() * sizeof(double), hipMemcpyHostToDevice, ln->st));
+  LAPH_HIP(hipMemcpyAsync(b->cts + ln->t0, hc.data(), hc.size() * sizeof(double), hipMemcpyHostToDevice, ln->st));
+  LAPH_HIP(launch_residual(v.X(), v.AX(), b->lam + v.nk(), b->cts + ln->t0, b->R32 + v.vo(), b->nrm2 + v.nk(), b->n,
+                           K, nt, ln->lock, ln->st));
+  LAPH_HIP(hipMemcpyAsync(b->h_res.data() + v.nk(), b->nrm2 + v.nk(), (size_t)nt * K * sizeof(double),
+                          hipMemcpyDeviceToHost, ln->st));
+  LAPH_HIP(hipStreamSynchronize(ln->st));
+  int active = 0, lmin = K;
+  for (int t = 0; t < nt; ++t) {
+    const double* res = &b->h_res[v.nk() + (size_t)t * K];
+    int nl = ln->lock;
+    while (nl < K && std::sqrt(res[nl]) <= b->o.tol) ++nl;
+    b->nlock[ln->t0 + t] = nl;
     if (nl < b->nev) ++active;
+    lmin = std::min(lmin, nl);
   }
   *n_active = active;
+  // hard-lock the common converged prefix (64-aligned), never past nev
+  if (active > 0) {
+    const int l = std::min(lmin, b->nev) / 64 * 64;
+    if (l > ln->lock) {
+      ln->lock = l;
+      std::vector<int> lb(nt, l / 64);
+      LAPH_HIP(hipMemcpyAsync(b->lock_blk + ln->t0, lb.data(), nt * sizeof(int), hipMemcpyHostToDevice, ln->st));
+      LAPH_HIP(hipStreamSynchronize(ln->st));
+    }
+  }
   return LAPH_OK;
 }
 
 // Per-column coefficients of the m-1 filter steps and the combine factor.
-static int upload_coefficients(laph_batch* b, const std::vector<double>& lc) {
-  const int K = b->K, nts = b->nts, m = b->deg;
+static int upload_coefficients(const View& v, const std::vector<double>& lc) {
+  laph_batch* b = v.b;
+  laph_lane* ln = v.ln;
+  const int K = b->K, nt = ln->nt, m = b->deg;
   const double lamL = b->p.lambda_l;
-  std::vector<float4> co((size_t)(m - 1) * nts * K);
-  std::vector<double> ga((size_t)nts * K);
-  std::vector<float2> cab(nts);
-  std::vector<int> fb(nts);
+  std::vector<float4> co((size_t)(m - 1) * nt * K);
+  std::vector<double> ga((size_t)nt * K);
+  std::vector<float2> cab(nt);
+  std::vector<int> fb(nt);
   std::vector<double> tau(m + 1), s(m + 1);
-  for (int t = 0; t < nts; ++t) {
+  for (int t = 0; t < nt; ++t) {
     const double c = 2.0 / (lamL - lc[t]);
     const double a = 1.0 + c * lc[t];
     cab[t] = make_float2((float)c, (float)(a - 6.0 * c));
-    const bool done = b->nlock[t] >= b->nev;
-    fb[t] = done ? K / 64 : b->nlock[t] / 64;
+    const int nl = b->nlock[ln->t0 + t];
+    fb[t] = nl >= b->nev ? K / 64 : nl / 64;
+    const double* mu = &b->h_mu[v.nk() + (size_t)t * K];
     for (int j = 0; j < K; ++j) {
-      const double th = 1.0 + c * (lc[t] - b->h_mu[(size_t)t * K + j]);  // Ritz value in B
+      const double th = 1.0 + c * (lc[t] - mu[j]);  // Ritz value in B
       tau[0] = 1.0;
       tau[1] = th;
       for (int k = 1; k < m; ++k) tau[k + 1] = 2.0 * th * tau[k] - tau[k - 1];
@@ -250,47 +358,53 @@ static int upload_coefficients(laph_batch* b, const std::vector<double>& lc) {
           q = make_float4((float)(2.0 * s[st] / s[st + 1]), (float)(2.0 * tau[st] / s[st + 1]),
                           (float)(-s[st - 1] / s[st + 1]), 0.f);
         }
-        co[((size_t)(st - 1) * nts + t) * K + j] = q;
+        co[((size_t)(st - 1) * nt + t) * K + j] = q;
       }
       ga[(size_t)t * K + j] = tau[m] / s[m];
     }
   }
-  LAPH_HIP(hipMemcpyAsync(b->coef, co.data(), co.size() * sizeof(float4), hipMemcpyHostToDevice, b->st));
-  LAPH_HIP(hipMemcpyAsync(b->gam, ga.data(), ga.size() * sizeof(double), hipMemcpyHostToDevice, b->st));
-  LAPH_HIP(hipMemcpyAsync(b->cab, cab.data(), cab.size() * sizeof(float2), hipMemcpyHostToDevice, b->st));
-  LAPH_HIP(hipMemcpyAsync(b->first_blk, fb.data(), fb.size() * sizeof(int), hipMemcpyHostToDevice, b->st));
+  // coefficient table of the lane: [step][lane timeslice][K] at offset step*nts*K + t0*K
+  for (int st = 1; st < m; ++st)
+    LAPH_HIP(hipMemcpyAsync(b->coef + (size_t)(st - 1) * b->nts * K + v.nk(), co.data() + (size_t)(st - 1) * nt * K,
+                            (size_t)nt * K * sizeof(float4), hipMemcpyHostToDevice, ln->st));
+  LAPH_HIP(hipMemcpyAsync(b->gam + v.nk(), ga.data(), ga.size() * sizeof(double), hipMemcpyHostToDevice, ln->st));
+  LAPH_HIP(hipMemcpyAsync(b->cab + ln->t0, cab.data(), cab.size() * sizeof(float2), hipMemcpyHostToDevice, ln->st));
+  LAPH_HIP(hipMemcpyAsync(b->first_blk + ln->t0, fb.data(), fb.size() * sizeof(int), hipMemcpyHostToDevice, ln->st));
   return LAPH_OK;
 }
 
-// The m-1 FP32 filter steps; buffers rotate R32 -> P0 -> P1 -> P2 -> P0 ...
-// Returns the buffer holding f_m.
-static int enqueue_filter(laph_batch* b, float2** fm) {
-  const int m = b->deg, K = b->K, nts = b->nts, L = b->p.L;
-  const size_t step = (size_t)nts * K;
+// The m-1 FP32 filter steps of a lane; buffers rotate R32 -> P0 -> P1 -> P2 -> P0 ...
+static int enqueue_filter(const View& v) {
+  laph_batch* b = v.b;
+  const int m = b->deg, K = b->K, nt = v.ln->nt, L = b->p.L;
+  const size_t step = (size_t)b->nts * K;
+  const size_t vo = v.vo();
+  float2* R = b->R32 + vo;
   for (int st = 1; st < m; ++st) {
-    const float2* in = st == 1 ? b->R32 : b->P[(st - 2) % 3];
-    const float2* old = st >= 3 ? b->P[(st - 3) % 3] : nullptr;
-    float2* out = b->P[(st - 1) % 3];
-    LAPH_HIP(launch_filter_step32(b->U32, in, b->R32, old, out, b->coef + (st - 1) * step, b->cab, b->first_blk,
-                                  L, K, nts, b->st));
+    const float2* in = st == 1 ? R : b->P[(st - 2) % 3] + vo;
+    const float2* old = st >= 3 ? b->P[(st - 3) % 3] + vo : nullptr;
+    float2* out = b->P[(st - 1) % 3] + vo;
+    LAPH_HIP(launch_filter_step32(b->U32 + v.lo(), in, R, old, out, b->coef + (st - 1) * step + v.nk(),
+                                  b->cab + v.ln->t0, b->first_blk + v.ln->t0, L, K, nt, v.ln->st));
   }
-  *fm = b->P[(m - 2) % 3];
   return LAPH_OK;
 }
 
-static int run_filter(laph_batch* b, float2** fm) {
-  if (!b->filter_graph) {
+// Replays the captured filter of the lane (captured on first use).
+static int run_filter(const View& v, float2** fm) {
+  laph_lane* ln = v.ln;
+  if (!ln->graph) {
     hipGraph_t g;
-    LAPH_HIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_filter(b, fm);
-    hipError_t e = hipStreamEndCapture(b->st, &g);
+    LAPH_HIP(hipStreamBeginCapture(ln->st, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_filter(v);
+    hipError_t e = hipStreamEndCapture(ln->st, &g);
     LAPH_TRY(rc);
     LAPH_HIP(e);
-    LAPH_HIP(hipGraphInstantiate(&b->filter_graph, g, nullptr, nullptr, 0));
+    LAPH_HIP(hipGraphInstantiate(&ln->graph, g, nullptr, nullptr, 0));
     LAPH_HIP(hipGraphDestroy(g));
   }
-  *fm = b->P[(b->deg - 2) % 3];
-  LAPH_HIP(hipGraphLaunch(b->filter_graph, b->st));
+  *fm = v.b->P[(v.b->deg - 2) % 3] + v.vo();
+  LAPH_HIP(hipGraphLaunch(ln->graph, ln->st));
   return LAPH_OK;
 }
 
@@ -308,84 +422,122 @@ __global__ void k_combine_gamma(double2* __restrict__ X, const float2* __restric
   X[idx] = make_double2(g * x.x + (double)f.x, g * x.y + (double)f.y);
 }
 
-static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
-  auto t0 = std::chrono::steady_clock::now();
-  const int K = b->K, nts = b->nts, L = b->p.L;
-  hipStream_t st = b->st;
-  b->iX = 0;
-  b->iAX = 1;
-  b->iT = 2;
-  // random start, orthonormalised twice, first Rayleigh-Ritz
-  LAPH_HIP(launch_random_block(b->B[b->iX], (size_t)nts * b->per_ts, b->o.seed, st));
-  LAPH_TRY(cholqr(b, b->B[b->iX]));
-  LAPH_TRY(cholqr(b, b->B[b->iX]));
-  LAPH_HIP(launch_lap_block64(b->U64, b->B[b->iX], b->B[b->iAX], L, K, nts, nullptr, st));
-  LAPH_TRY(rayleigh_ritz(b));
-  std::vector<double> lc(nts);
+// Solve loop of one lane (runs on its own host thread).
+static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
+  const View v{b, ln};
+  const int K = b->K, nt = ln->nt;
+  ln->iX = 0;
+  ln->iAX = 1;
+  ln->iT = 2;
+  ln->lock = 0;
+  ln->cycles = 0;
+  ln->sv32 = ln->sv64 = ln->flaunch = 0;
+  ln->fbytes = 0;
+  ln->ms_filter = ln->ms_dense = 0;
+  {
+    std::vector<int> zero(nt, 0);
+    LAPH_HIP(hipMemcpyAsync(b->lock_blk + ln->t0, zero.data(), nt * sizeof(int), hipMemcpyHostToDevice, ln->st));
+  }
+  // start block (network proposal or seeded Gaussian), CholQR2, first Rayleigh-Ritz
+  if (!warm) LAPH_HIP(launch_random_block(v.X(), (size_t)nt * b->per_ts, b->o.seed + 0x1000193ull * ln->t0, ln->st));
+  LAPH_TRY(orthonormalise_and_rr(v, Act{0, K}, true));
+  std::vector<double> lc(nt);
   int active = 0;
-  LAPH_TRY(residuals_and_window(b, lc, &active));
-  long long sv32 = 0, sv64 = (long long)nts * K, flaunch = 0;
-  double fbytes = 0;
-  float ms_filter = 0, ms_dense = 0;
+  LAPH_TRY(residuals_and_window(v, lc, &active));
   const int max_iter = b->o.max_iter > 0 ? b->o.max_iter : 60;
   int it = 0;
   for (it = 0; it < max_iter && active > 0; ++it) {
-    LAPH_TRY(upload_coefficients(b, lc));
-    for (int t = 0; t < nts; ++t) {
-      if (b->nlock[t] >= b->nev) continue;
-      const int cols = K - (b->nlock[t] / 64) * 64;
-      sv32 += (long long)(b->deg - 1) * cols;
+    LAPH_TRY(upload_coefficients(v, lc));
+    for (int t = 0; t < nt; ++t) {
+      const int nl = b->nlock[ln->t0 + t];
+      if (nl >= b->nev) continue;
+      const int cols = K - (nl / 64) * 64;
+      ln->sv32 += (long long)(b->deg - 1) * cols;
       // algorithmic bytes: step 1 reads r and writes f_2 (2 vector touches),
       // step 2 adds r (3), later steps add f_{k-1} (4); a vector touch is
       // 3 colours x 8 B per site; links 27 x 8 B per site per 64-vector block
       for (int s = 1; s < b->deg; ++s) {
         const int touches = s == 1 ? 2 : (s == 2 ? 3 : 4);
-        fbytes += (double)b->V3 * ((double)cols * touches * 24.0 + (cols / 64) * 216.0);
+        ln->fbytes += (double)b->V3 * ((double)cols * touches * 24.0 + (cols / 64) * 216.0);
       }
     }
-    flaunch += b->deg - 1;
+    ln->flaunch += b->deg - 1;
     float2* fm = nullptr;
-    LAPH_HIP(hipEventRecord(b->ev[0], st));
-    LAPH_TRY(run_filter(b, &fm));
-    LAPH_HIP(hipEventRecord(b->ev[1], st));
-    dim3 g((unsigned)((b->per_ts + 255) / 256), nts);
-    k_combine_gamma<<<g, 256, 0, st>>>(b->B[b->iX], fm, b->gam, b->per_ts, K, b->first_blk);
+    LAPH_HIP(hipEventRecord(ln->ev[0], ln->st));
+    LAPH_TRY(run_filter(v, &fm));
+    LAPH_HIP(hipEventRecord(ln->ev[1], ln->st));
+    dim3 g((unsigned)((b->per_ts + 255) / 256), nt);
+    k_combine_gamma<<<g, 256, 0, ln->st>>>(v.X(), fm, b->gam + v.nk(), b->per_ts, K, b->first_blk + ln->t0);
     LAPH_HIP(hipGetLastError());
-    LAPH_TRY(cholqr(b, b->B[b->iX]));
-    LAPH_TRY(cholqr(b, b->B[b->iX]));
-    LAPH_HIP(launch_lap_block64(b->U64, b->B[b->iX], b->B[b->iAX], L, K, nts, nullptr, st));
-    sv64 += (long long)nts * K;
-    LAPH_TRY(rayleigh_ritz(b));
-    LAPH_HIP(hipEventRecord(b->ev[2], st));
-    LAPH_TRY(residuals_and_window(b, lc, &active));
+    const Act q{ln->lock, K - ln->lock};
+    LAPH_TRY(project_locked(v, q));
+    // explicit CholQR2 while the block is far from orthonormal (first
+    // cycles from a random start), Cholesky inside Rayleigh-Ritz after
+    LAPH_TRY(orthonormalise_and_rr(v, q, it < 2));
+    LAPH_HIP(hipEventRecord(ln->ev[2], ln->st));
+    LAPH_TRY(residuals_and_window(v, lc, &active));
     float a = 0, c = 0;
-    LAPH_HIP(hipEventElapsedTime(&a, b->ev[0], b->ev[1]));
-    LAPH_HIP(hipEventElapsedTime(&c, b->ev[1], b->ev[2]));
-    ms_filter += a;
-    ms_dense += c;
+    LAPH_HIP(hipEventElapsedTime(&a, ln->ev[0], ln->ev[1]));
+    LAPH_HIP(hipEventElapsedTime(&c, ln->ev[1], ln->ev[2]));
+    ln->ms_filter += a;
+    ln->ms_dense += c;
   }
+  ln->cycles = it;
+  // eigenvectors of every lane end in B[0]
+  if (ln->iX != 0)
+    LAPH_HIP(hipMemcpyAsync(v.blk(0), v.X(), (size_t)nt * b->per_ts * sizeof(double2), hipMemcpyDeviceToDevice,
+                            ln->st));
+  LAPH_HIP(hipStreamSynchronize(ln->st));
+  ln->iX = 0;
+  return LAPH_OK;
+}
+
+static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
+  auto t0 = std::chrono::steady_clock::now();
+  const bool warm = b->o.warm_start && b->have_start;
+  b->have_start = false;
+  b->solved = false;
+  LAPH_HIP(hipStreamSynchronize(b->st));
+  const int dev = b->p.device;
+  std::vector<std::thread> th;
+  for (auto& ln : b->lanes)
+    th.emplace_back([b, &ln, warm, dev] {
+      if (hipSetDevice(dev) != hipSuccess) {
+        ln.rc = LAPH_ERR_HIP;
+        ln.msg = "lane thread: hipSetDevice failed";
+        return;
+      }
+      ln.rc = guarded([&] { return lane_solve(b, &ln, warm); });
+      if (ln.rc != LAPH_OK) ln.msg = laph_last_error();
+    });
+  for (auto& t : th) t.join();
+  for (auto& ln : b->lanes)
+    if (ln.rc != LAPH_OK) return set_error(ln.rc, "%s", ln.msg.c_str());
   b->solved = true;
-  int conv = 0;
+  int conv = 0, cycles = 0;
   double maxres = 0;
-  for (int t = 0; t < nts; ++t) {
+  laph_batch_stats s{};
+  for (int t = 0; t < b->nts; ++t) {
     if (b->nlock[t] >= b->nev) ++conv;
-    for (int j = 0; j < b->nev; ++j) maxres = std::max(maxres, std::sqrt(b->h_res[(size_t)t * K + j]));
+    for (int j = 0; j < b->nev; ++j) maxres = std::max(maxres, std::sqrt(b->h_res[(size_t)t * b->K + j]));
   }
-  if (stats) {
-    stats->iterations = it;
-    stats->stencil_vecs = sv32;
-    stats->stencil64_vecs = sv64;
-    stats->converged = conv;
-    stats->max_residual = maxres;
-    stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    stats->seconds_filter = ms_filter * 1e-3;
-    stats->seconds_dense = ms_dense * 1e-3;
-    stats->filter_launches = flaunch;
-    stats->filter_bytes = fbytes;
+  for (auto& ln : b->lanes) {
+    cycles = std::max(cycles, ln.cycles);
+    s.stencil_vecs += ln.sv32;
+    s.stencil64_vecs += ln.sv64;
+    s.filter_launches += ln.flaunch;
+    s.filter_bytes += ln.fbytes;
+    s.seconds_filter += ln.ms_filter * 1e-3;
+    s.seconds_dense += ln.ms_dense * 1e-3;
   }
-  if (conv < nts)
-    return set_error(LAPH_ERR_NOCONV, "laph_batch_solve: %d of %d timeslices converged after %d cycles", conv, nts,
-                     it);
+  s.iterations = cycles;
+  s.converged = conv;
+  s.max_residual = maxres;
+  s.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (stats) *stats = s;
+  if (conv < b->nts)
+    return set_error(LAPH_ERR_NOCONV, "laph_batch_solve: %d of %d timeslices converged after %d cycles", conv, b->nts,
+                     cycles);
   return LAPH_OK;
 }
 
@@ -429,8 +581,6 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     };
     if (hipSetDevice(p->device) != hipSuccess || hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess)
       return fail(laph::set_error(LAPH_ERR_HIP, "laph_batch_create: stream creation failed"));
-    if (rocblas_create_handle(&b->rb) != rocblas_status_success || rocblas_set_stream(b->rb, b->st) != rocblas_status_success)
-      return fail(laph::set_error(LAPH_ERR_HIP, "laph_batch_create: rocBLAS handle"));
     int rc = laph::batch_alloc(b);
     if (rc != LAPH_OK) return fail(rc);
     *out = b;
@@ -486,6 +636,38 @@ int laph_batch_solve(laph_batch* b, laph_batch_stats* stats) {
   return laph::guarded([&] { return laph::solve_impl(b, stats); });
 }
 
+int laph_batch_time_filter(laph_batch* b, int reps, double* ms_per_launch, double* bytes_per_launch) {
+  LAPH_CHECK(b && reps >= 1 && ms_per_launch && bytes_per_launch, LAPH_ERR_INVALID,
+             "laph_batch_time_filter: bad arguments");
+  LAPH_CHECK(b->solved, LAPH_ERR_INVALID, "laph_batch_time_filter: solve the batch first (coefficient tables)");
+  LAPH_HIP(hipSetDevice(b->p.device));
+  return laph::guarded([&]() -> int {
+    laph_lane* ln = &b->lanes[0];
+    const laph::View v{b, ln};
+    const int K = b->K, nt = ln->nt;
+    // every column of every timeslice of lane 0 active
+    std::vector<int> zero(nt, 0);
+    LAPH_HIP(hipMemcpyAsync(b->first_blk + ln->t0, zero.data(), nt * sizeof(int), hipMemcpyHostToDevice, ln->st));
+    float2* fm = nullptr;
+    LAPH_TRY(laph::run_filter(v, &fm));  // warm-up (and capture if needed)
+    LAPH_HIP(hipEventRecord(ln->ev[0], ln->st));
+    for (int r = 0; r < reps; ++r) LAPH_TRY(laph::run_filter(v, &fm));
+    LAPH_HIP(hipEventRecord(ln->ev[1], ln->st));
+    LAPH_HIP(hipEventSynchronize(ln->ev[1]));
+    float ms = 0;
+    LAPH_HIP(hipEventElapsedTime(&ms, ln->ev[0], ln->ev[1]));
+    const int steps = b->deg - 1;
+    double bytes = 0;
+    for (int s = 1; s < b->deg; ++s) {
+      const int touches = s == 1 ? 2 : (s == 2 ? 3 : 4);
+      bytes += (double)nt * b->V3 * ((double)K * touches * 24.0 + (K / 64) * 216.0);
+    }
+    *ms_per_launch = ms / ((double)reps * steps);
+    *bytes_per_launch = bytes / steps;
+    return LAPH_OK;
+  });
+}
+
 int laph_batch_get(laph_batch* b, int ts, double* evals, double* V_fix, double* phase, double* residuals) {
   LAPH_CHECK(b && ts >= 0 && ts < b->nts && evals, LAPH_ERR_INVALID, "laph_batch_get: bad arguments");
   LAPH_CHECK(b->solved, LAPH_ERR_INVALID, "laph_batch_get: batch not solved");
@@ -499,8 +681,7 @@ int laph_batch_get(laph_batch* b, int ts, double* evals, double* V_fix, double* 
     double* dph = nullptr;
     LAPH_HIP(hipMalloc(&dV, (size_t)b->n * nev * sizeof(double2)));
     hipError_t e = hipMalloc(&dph, nev * sizeof(double));
-    if (e == hipSuccess)
-      e = laph::launch_extract_phase(b->B[b->iX] + (size_t)ts * b->per_ts, dV, dph, b->n, K, nev, b->st);
+    if (e == hipSuccess) e = laph::launch_extract_phase(b->B[0] + (size_t)ts * b->per_ts, dV, dph, b->n, K, nev, b->st);
     if (e == hipSuccess && V_fix)
       e = hipMemcpyAsync(V_fix, dV, (size_t)b->n * nev * sizeof(double2), hipMemcpyDeviceToHost, b->st);
     if (e == hipSuccess && phase) e = hipMemcpyAsync(phase, dph, nev * sizeof(double), hipMemcpyDeviceToHost, b->st);

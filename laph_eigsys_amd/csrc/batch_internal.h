@@ -1,0 +1,77 @@
+// State of a laph_batch (the mixed-precision batched solver), shared by
+// batch.hip and the warm-start network (net.hip).
+#pragma once
+
+#include <rocblas/rocblas.h>
+
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+#define LAPH_RB(call)                                                                               \
+  do {                                                                                              \
+    rocblas_status s_ = (call);                                                                     \
+    if (s_ != rocblas_status_success)                                                               \
+      return laph::set_error(LAPH_ERR_HIP, "%s failed: %s", #call, rocblas_status_to_string(s_));   \
+  } while (0)
+
+// One lane = a contiguous range of timeslices [t0, t0+nt) solved by its own
+// host thread on its own HIP stream and rocBLAS handle.  Two lanes run side
+// by side so the HBM-bound FP32 filter of one overlaps the MFMA-bound dense
+// FP64 steps of the other.
+struct laph_lane {
+  int t0 = 0, nt = 0;
+  hipStream_t st = nullptr;
+  rocblas_handle rb = nullptr;
+  int iX = 0, iAX = 1, iT = 2;  // roles of the three FP64 block buffers
+  int lock = 0;                 // hard-locked columns, common to the lane
+  hipGraphExec_t graph = nullptr;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  // per-solve statistics
+  int cycles = 0, rc = 0;
+  std::string msg;  // error message raised on the lane's thread
+  long long sv32 = 0, sv64 = 0, flaunch = 0;
+  double fbytes = 0;
+  float ms_filter = 0, ms_dense = 0;
+};
+
+struct laph_batch {
+  laph_params p{};
+  laph_batch_opts o{};
+  int V3 = 0, n = 0, K = 0, nts = 0, nev = 0, deg = 0;
+  size_t per_ts = 0;  // n*K
+  hipStream_t st = nullptr;  // set-up / extraction stream
+  // links
+  double2* U64 = nullptr;  // [nts][V3*27]
+  float2* U32 = nullptr;
+  double2* dec = nullptr;   // HYP scratch for one timeslice
+  double2* Unew = nullptr;
+  // blocks [nts][n][K]; after a solve the eigenvectors are in B[0]
+  double2* B[3] = {nullptr, nullptr, nullptr};
+  float2* R32 = nullptr;
+  float2* P[3] = {nullptr, nullptr, nullptr};
+  // small per-timeslice data
+  double2* G = nullptr;      // [nts][K*K] Rayleigh-Ritz / CholQR matrix
+  double2* G2 = nullptr;     // [nts][K*K] Gram of the active block
+  double2* P2 = nullptr;     // [nts][K*K] projections on the locked block
+  double* mu = nullptr;      // [nts][K]
+  double* E = nullptr;       // [nts][K]
+  rocblas_int* info = nullptr;
+  double* nrm2 = nullptr;    // [nts][K]
+  double* cn = nullptr;      // [nts][K] column norms for CholQR scaling
+  double* lam = nullptr;     // [nts][K]
+  double* cts = nullptr;     // [nts]
+  double* gam = nullptr;     // [nts][K]
+  float4* coef = nullptr;    // [deg-1][nts][K]
+  float2* cab = nullptr;     // [nts]
+  int* first_blk = nullptr;  // [nts] first filtered 64-column block
+  int* lock_blk = nullptr;   // [nts] first active 64-column block of the dense steps
+  // host mirrors, written by the lanes on disjoint ranges
+  std::vector<int> nlock;
+  std::vector<double> h_mu, h_res;
+  std::vector<laph_lane> lanes;
+  bool solved = false;
+  bool have_start = false;  // B[0] holds a proposed start block (laph_net_propose)
+};
+

@@ -18,12 +18,16 @@ hipError_t launch_lap_block64(const double2* U64, const double2* v, double2* out
 // X += F on active columns.
 hipError_t launch_combine(double2* X, const float2* F, size_t per_ts, int K, int nts, const int* first_blk,
                           hipStream_t st);
-// R = AX - X diag(lam), r32 = c_t R, nrm2 = column sums of |R|^2.
+// R = AX - X diag(lam), r32 = c_t R, nrm2 = column sums of |R|^2, for the
+// columns >= first_col (a multiple of 64).
 hipError_t launch_residual(const double2* X, const double2* AX, const double* lam, const double* cts,
-                           float2* r32, double* nrm2, int n, int K, int nts, hipStream_t st);
+                           float2* r32, double* nrm2, int n, int K, int nts, int first_col, hipStream_t st);
 // Column-major, phase-fixed copy of the first nev columns of one timeslice.
 hipError_t launch_extract_phase(const double2* X, double2* V, double* phase, int n, int K, int nev,
                                 hipStream_t st);
+// Scales every column >= first_col (a multiple of 64) of a [ts][n][K] block to
+// unit norm; tmp holds nts*K doubles.
+hipError_t launch_normalize_columns(double2* Y, double* tmp, int n, int K, int nts, int first_col, hipStream_t st);
 hipError_t launch_convert_links32(const double2* U, float2* U32, size_t n, hipStream_t st);
 hipError_t launch_random_block(double2* X, size_t total, unsigned long long seed, hipStream_t st);
 

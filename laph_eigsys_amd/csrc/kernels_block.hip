@@ -171,17 +171,18 @@ __global__ void k_combine(double2* __restrict__ X, const float2* __restrict__ F,
   X[idx] = x;
 }
 
-// R = AX - X diag(lam); r32 = c_t * R (B-space residual, FP32); nrm2[ts][col] += |R|^2.
-// grid (row chunks, K/64 column blocks, ts); each thread sums ROWS rows of one column.
+// R = AX - X diag(lam); r32 = c_t * R (B-space residual, FP32); nrm2[ts][col] += |R|^2,
+// for columns >= first_col.  grid (row chunks, active 64-column blocks, ts);
+// each thread sums the rows of one column within its chunk.
 __global__ __launch_bounds__(256) void k_residual(const double2* __restrict__ X,
                                                    const double2* __restrict__ AX,
                                                    const double* __restrict__ lam,
                                                    const double* __restrict__ cts,
                                                    float2* __restrict__ r32,
                                                    double* __restrict__ nrm2, int n, int K,
-                                                   int rows_per_block) {
+                                                   int rows_per_block, int first_col) {
   const int ts = blockIdx.z;
-  const int col = blockIdx.y * 64 + (threadIdx.x & 63);
+  const int col = first_col + blockIdx.y * 64 + (threadIdx.x & 63);
   const int r0 = blockIdx.x * rows_per_block + (threadIdx.x >> 6);
   const int r1 = min(n, (int)(blockIdx.x + 1) * rows_per_block);
   const double l = lam[(size_t)ts * K + col];
@@ -204,9 +205,12 @@ __global__ __launch_bounds__(256) void k_residual(const double2* __restrict__ X,
   }
 }
 
-__global__ void k_fill_zero_d(double* __restrict__ p, size_t n) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0.0;
+// Zero nrm2[ts][col] for col >= first_col.
+__global__ void k_zero_cols(double* __restrict__ p, int nts, int K, int first_col) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = K - first_col;
+  if (i >= nts * w) return;
+  p[(size_t)(i / w) * K + first_col + i % w] = 0.0;
 }
 
 // Output: V[col][row] (column-major n x nev) = e^{-i phi_col} X[row][col],
@@ -274,12 +278,12 @@ hipError_t launch_combine(double2* X, const float2* F, size_t per_ts, int K, int
 }
 
 hipError_t launch_residual(const double2* X, const double2* AX, const double* lam, const double* cts,
-                           float2* r32, double* nrm2, int n, int K, int nts, hipStream_t st) {
-  const size_t nn = (size_t)nts * K;
-  k_fill_zero_d<<<(unsigned)((nn + 255) / 256), 256, 0, st>>>(nrm2, nn);
+                           float2* r32, double* nrm2, int n, int K, int nts, int first_col, hipStream_t st) {
+  const int w = K - first_col;
+  k_zero_cols<<<(nts * w + 255) / 256, 256, 0, st>>>(nrm2, nts, K, first_col);
   const int rows = 256;
-  dim3 g((n + rows - 1) / rows, K / 64, nts);
-  k_residual<<<g, 256, 0, st>>>(X, AX, lam, cts, r32, nrm2, n, K, rows);
+  dim3 g((n + rows - 1) / rows, w / 64, nts);
+  k_residual<<<g, 256, 0, st>>>(X, AX, lam, cts, r32, nrm2, n, K, rows, first_col);
   return hipGetLastError();
 }
 
@@ -297,6 +301,55 @@ hipError_t launch_convert_links32(const double2* U, float2* U32, size_t n, hipSt
 
 hipError_t launch_random_block(double2* X, size_t total, unsigned long long seed, hipStream_t st) {
   k_random_block<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(X, total, seed);
+  return hipGetLastError();
+}
+
+}  // namespace laph
+
+namespace laph {
+
+// nrm[ts][col] = sum_rows |Y[ts][row][col]|^2 for col >= first_col (zeroed first).
+__global__ __launch_bounds__(256) void k_colnorm2_blk(const double2* __restrict__ Y, double* __restrict__ nrm, int n,
+                                                       int K, int first_col) {
+  const int ts = blockIdx.z;
+  const int col = first_col + blockIdx.y * 64 + (threadIdx.x & 63);
+  const int r0 = blockIdx.x * 256 + (threadIdx.x >> 6), r1 = min(n, (int)(blockIdx.x + 1) * 256);
+  const size_t off = (size_t)ts * n * K;
+  double acc = 0.0;
+  for (int r = r0; r < r1; r += 4) {
+    const double2 v = Y[off + (size_t)r * K + col];
+    acc += v.x * v.x + v.y * v.y;
+  }
+  __shared__ double part[256];
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < 64)
+    atomicAdd(&nrm[(size_t)ts * K + col],
+              part[threadIdx.x] + part[threadIdx.x + 64] + part[threadIdx.x + 128] + part[threadIdx.x + 192]);
+}
+
+// Y[ts][row][col] /= sqrt(nrm[ts][col]) for col >= first_col.
+__global__ void k_scale_cols(double2* __restrict__ Y, const double* __restrict__ nrm, size_t per_ts, int K,
+                             int first_col) {
+  const int ts = blockIdx.y;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= per_ts) return;
+  const int col = (int)(i % K);
+  if (col < first_col) return;
+  const double nv = nrm[(size_t)ts * K + col];
+  const double s = nv > 0.0 ? 1.0 / sqrt(nv) : 1.0;
+  double2 v = Y[(size_t)ts * per_ts + i];
+  Y[(size_t)ts * per_ts + i] = make_double2(v.x * s, v.y * s);
+}
+
+hipError_t launch_normalize_columns(double2* Y, double* tmp, int n, int K, int nts, int first_col, hipStream_t st) {
+  const int w = K - first_col;
+  k_zero_cols<<<(nts * w + 255) / 256, 256, 0, st>>>(tmp, nts, K, first_col);
+  dim3 g((n + 255) / 256, w / 64, nts);
+  k_colnorm2_blk<<<g, 256, 0, st>>>(Y, tmp, n, K, first_col);
+  const size_t per_ts = (size_t)n * K;
+  dim3 gs((unsigned)((per_ts + 255) / 256), nts);
+  k_scale_cols<<<gs, 256, 0, st>>>(Y, tmp, per_ts, K, first_col);
   return hipGetLastError();
 }
 

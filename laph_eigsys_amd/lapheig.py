@@ -98,6 +98,16 @@ SIGNATURES = {
     "laph_batch_set_links": ([_VP, _I, _dp], _I),
     "laph_batch_solve": ([_VP, ctypes.POINTER(laph_batch_stats)], _I),
     "laph_batch_get": ([_VP, _I, _dp, _dp, _dp, _dp], _I),
+    "laph_batch_time_filter": ([_VP, _I, _dp, _dp], _I),
+    "laph_net_create": ([ctypes.POINTER(_VP), _I, ctypes.c_ulonglong, _I], _I),
+    "laph_net_destroy": ([_VP], _I),
+    "laph_net_num_params": ([_VP], _I),
+    "laph_net_get_params": ([_VP, ctypes.POINTER(ctypes.c_float)], _I),
+    "laph_net_set_params": ([_VP, ctypes.POINTER(ctypes.c_float)], _I),
+    "laph_net_propose": ([_VP, _VP, ctypes.c_ulonglong], _I),
+    "laph_net_train_step": ([_VP, _VP, ctypes.c_ulonglong, _D, _dp], _I),
+    "laph_net_loss_grad": ([_VP, _I, _I, _I, _dp, ctypes.POINTER(ctypes.c_float), _dp, _I, _dp, _dp,
+                            ctypes.POINTER(ctypes.c_float)], _I),
 }
 
 _lib = None
@@ -293,11 +303,11 @@ class Batch:
     """All timeslices of a configuration in one mixed-precision batched solve."""
 
     def __init__(self, L, deg, lambda_l, lambda_c, nts, nev, nsub=0, tol=1e-10, max_iter=0, seed=1227,
-                 device=0):
+                 device=0, warm_start=0):
         self.L, self.nts, self.nev = L, nts, nev
         self.n = 3 * L ** 3
         self.params = laph_params(L, deg, lambda_l, lambda_c, device)
-        self.opts = laph_batch_opts(nts, nev, nsub, tol, max_iter, seed, 0)
+        self.opts = laph_batch_opts(nts, nev, nsub, tol, max_iter, seed, int(warm_start))
         self.h = ctypes.c_void_p()
         _check(lib().laph_batch_create(ctypes.byref(self.h), ctypes.byref(self.params),
                                        ctypes.byref(self.opts)), "laph_batch_create")
@@ -327,6 +337,12 @@ class Batch:
         _check(lib().laph_batch_solve(self.h, ctypes.byref(st)), "laph_batch_solve")
         return st
 
+    def time_filter(self, reps=20):
+        """(ms per filter-step launch, algorithmic bytes per launch), kernel alone on the device."""
+        ms, by = ctypes.c_double(), ctypes.c_double()
+        _check(lib().laph_batch_time_filter(self.h, reps, ctypes.byref(ms), ctypes.byref(by)), "laph_batch_time_filter")
+        return ms.value, by.value
+
     def get(self, ts, want_vectors=True):
         ev = np.empty(self.nev)
         ph = np.empty(self.nev)
@@ -335,3 +351,66 @@ class Batch:
         _check(lib().laph_batch_get(self.h, ts, _p(ev), None if V is None else _p(V), _p(ph), _p(res)),
                "laph_batch_get")
         return ev, V, ph, res
+
+
+def _fp(a):
+    assert a.flags["C_CONTIGUOUS"] and a.dtype == np.float32
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Net:
+    """Gauge-covariant warm-start network (16 complex channels, BF16
+    activations): proposes the start block of a Batch."""
+
+    def __init__(self, layers=4, seed=1, device=0):
+        self.layers = layers
+        self.h = ctypes.c_void_p()
+        _check(lib().laph_net_create(ctypes.byref(self.h), layers, seed, device), "laph_net_create")
+
+    def close(self):
+        if self.h:
+            lib().laph_net_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def num_params(self):
+        return lib().laph_net_num_params(self.h)
+
+    def get_params(self):
+        p = np.empty(self.num_params, dtype=np.float32)
+        _check(lib().laph_net_get_params(self.h, _fp(p)), "laph_net_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.num_params
+        _check(lib().laph_net_set_params(self.h, _fp(p)), "laph_net_set_params")
+
+    def propose(self, batch, seed):
+        _check(lib().laph_net_propose(self.h, batch.h, seed), "laph_net_propose")
+
+    def train_step(self, batch, seed, lr):
+        loss = ctypes.c_double()
+        _check(lib().laph_net_train_step(self.h, batch.h, seed, lr, ctypes.byref(loss)), "laph_net_train_step")
+        return loss.value
+
+    def loss_grad(self, L, links, x, V, fp32_act=True):
+        """links (V3,3,3,3) complex; x (n, K) complex; V (n, nev) complex."""
+        links = _c128(links).reshape(-1)
+        n, K = x.shape
+        xin = np.ascontiguousarray(x, dtype=np.complex64)
+        xf = xin.view(np.float32)
+        Vc = _c128(V)
+        nev = Vc.shape[1]
+        loss = ctypes.c_double()
+        y = np.empty((n, K), dtype=np.complex128)
+        grad = np.empty(self.num_params, dtype=np.float32)
+        _check(lib().laph_net_loss_grad(self.h, L, K, int(bool(fp32_act)), _p(links), _fp(xf.reshape(-1)), _p(Vc),
+                                        nev, ctypes.byref(loss), _p(y), _fp(grad)), "laph_net_loss_grad")
+        return loss.value, y, grad
