@@ -49,6 +49,8 @@ def main():
     ap.add_argument("--plain-ts", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--filter-reps", type=int, default=20)
+    ap.add_argument("--profile-filter", action="store_true",
+                    help="counter-collection workload: filter replays + a calibration copy, no solve timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); timeslices are "
@@ -56,6 +58,9 @@ def main():
         sys.exit(2)
 
     from laph_eigsys_amd import lapheig as lp
+    if args.profile_filter:
+        profile_filter(lp, args)
+        return
     nts = args.nts
     t0 = time.time()
     cfg = lp.synth_config(SEED, EPS, L, nts)
@@ -132,6 +137,35 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def profile_filter(lp, args):
+    """Counter-collection workload: one batched solve of --nts timeslices (for
+    the coefficient tables), then --filter-reps replays of the captured FP32
+    filter alone, then a 1 GiB device-to-device copy whose FETCH_SIZE /
+    WRITE_SIZE calibrate the counters (2 x 1 GiB of known traffic)."""
+    import ctypes
+    nts = args.nts
+    cfg = lp.synth_config(SEED, EPS, L, nts)
+    vol = 72 * L ** 3
+    b = lp.Batch(L, DEG, LAMBDA_L, 0.0, nts, NEV, nsub=NSUB, tol=TOL, seed=SEED)
+    for t in range(nts):
+        b.set_timeslice(t, cfg[t * vol:(t + 1) * vol], 0.62, 0.62, 3)
+    b.solve()
+    ms, by = b.time_filter(reps=args.filter_reps)
+    b.close()
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = 1 << 30
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"filter_ms_per_launch": ms, "filter_bytes_per_launch": by, "filter_GBps": by / ms / 1e6,
+                      "lane_timeslices": (nts + 1) // 2, "copy_bytes": nbytes}))
 
 
 def device_sync():

@@ -417,7 +417,8 @@ __global__ void k_combine_gamma(double2* __restrict__ X, const float2* __restric
   if (col < first_blk[ts] * 64) return;
   const size_t idx = (size_t)ts * per_ts + i;
   const double g = gam[(size_t)ts * K + col];
-  const float2 f = F[idx];
+  const size_t row = i / K;  // F is block-major: [ts][col/64][row][64]
+  const float2 f = F[(size_t)ts * per_ts + ((size_t)(col >> 6) * (per_ts / K) + row) * 64 + (col & 63)];
   const double2 x = X[idx];
   X[idx] = make_double2(g * x.x + (double)f.x, g * x.y + (double)f.y);
 }

@@ -15,9 +15,6 @@ hipError_t launch_filter_step32(const float2* U32, const float2* in, const float
 // out = D v on every (or, with first_blk, every active) column.
 hipError_t launch_lap_block64(const double2* U64, const double2* v, double2* out, int L, int K, int nts,
                               const int* first_blk, hipStream_t st);
-// X += F on active columns.
-hipError_t launch_combine(double2* X, const float2* F, size_t per_ts, int K, int nts, const int* first_blk,
-                          hipStream_t st);
 // R = AX - X diag(lam), r32 = c_t R, nrm2 = column sums of |R|^2, for the
 // columns >= first_col (a multiple of 64).
 hipError_t launch_residual(const double2* X, const double2* AX, const double* lam, const double* cts,

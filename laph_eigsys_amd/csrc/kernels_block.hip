@@ -1,6 +1,9 @@
-// Blocked multi-vector kernels of the batched solver.  A block of vectors of
-// one timeslice is stored site-major with the vectors fastest:
+// Blocked multi-vector kernels of the batched solver.  The FP64 blocks of one
+// timeslice are stored site-major with the vectors fastest:
 //     X[ts][(s*3 + colour)*K + col]            (K a multiple of 64)
+// (the layout the strided-batched GEMMs read as a K x n matrix), the FP32
+// filter buffers block-major, one contiguous array per 64-vector block:
+//     F[ts][col/64][(s*3 + colour)*64 + col%64]
 // so one 64-lane wavefront holds 64 vectors of ONE site: the links of that
 // site are wave-uniform and are fetched once per wavefront through scalar
 // loads (the link matrix is read once per 64 vectors), while every vector
@@ -8,32 +11,57 @@
 // Links per timeslice are AoS [s][dir][9] so a site's 3 forward matrices are
 // one contiguous 216 B (FP32) / 432 B (FP64) scalar fetch.
 //
-// Work-group = 4 wavefronts = 4 consecutive sites of one 64-vector block.
-// Workgroups are handed to the 8 XCDs round-robin; the site-block index is
-// remapped so each XCD sweeps one contiguous slab of sites, which keeps the
-// x-neighbour plane (L*L sites away) in that XCD's L2.
+// Work-group = 4 wavefronts = 4 consecutive sites of one 64-vector block of
+// one timeslice (a "unit" = one (timeslice, 64-vector block) pair).
+// Workgroups are handed to the 8 XCDs round-robin by linear id; the 1D grid
+// is decoded so that every XCD owns whole units and sweeps each unit plane by
+// plane (site order x-major).  A vector is then reused as the x-, y- and
+// z-neighbour of later sites while it is still in that XCD's L2: the reuse
+// distance is two x-planes (2 * L*L sites * 1.5 KiB).  The streamed arrays
+// (r, f_{k-1}, the output) are accessed non-temporally so they do not evict
+// those planes.
 #include <hip/hip_runtime.h>
 
 #include "kernels_block.h"
 
 namespace laph {
 
-__device__ __forceinline__ int xcd_remap(int bid, int nb) {
-  if ((nb & 7) != 0) return bid;
-  return (bid & 7) * (nb >> 3) + (bid >> 3);
+// Decodes blockIdx.x of a 1D grid of SB site-blocks x nb column blocks x nts
+// timeslices into (ts, cb, sb), giving each XCD (linear id mod 8) whole units.
+__device__ __forceinline__ void unit_map(int SB, int nb, int nts, int& ts, int& cb, int& sb) {
+  const unsigned id = blockIdx.x;
+  const unsigned U = (unsigned)nb * nts;
+  unsigned unit;
+  if ((U & 7u) == 0) {
+    const unsigned j = id >> 3;
+    sb = (int)(j % SB);
+    unit = (j / SB) * 8 + (id & 7u);
+  } else {
+    sb = (int)(id % SB);
+    unit = id / SB;
+  }
+  ts = (int)(unit / nb);
+  cb = (int)(unit % nb);
 }
 
-template <class C>
-struct cplx_ops;
-
-template <>
-struct cplx_ops<float2> {
-  using real = float;
-};
-template <>
-struct cplx_ops<double2> {
-  using real = double;
-};
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 ld_nt(const float2* p) {
+  const f2v v = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(p));
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void st_nt(float2* p, float2 v) {
+  f2v w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<f2v*>(p));
+}
+__device__ __forceinline__ void st_nt(double2* p, double2 v) {
+  d2v w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<d2v*>(p));
+}
 
 template <class C>
 __device__ __forceinline__ C cfma(C u, C v, C acc) {  // acc += u*v
@@ -96,51 +124,53 @@ __global__ __launch_bounds__(256) void k_filter_step32(const float2* __restrict_
                                                         const float4* __restrict__ coef,
                                                         const float2* __restrict__ cab,
                                                         const int* __restrict__ first_blk, int L,
-                                                        int K, int use_old) {
-  const int ts = blockIdx.z;
-  const int cb = blockIdx.y;
-  if (cb < first_blk[ts]) return;
+                                                        int K, int nts, int use_old) {
   const int V3 = L * L * L;
+  int ts, cb, sb;
+  unit_map((V3 + 3) / 4, K / 64, nts, ts, cb, sb);
+  if (cb < first_blk[ts]) return;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int s = xcd_remap(blockIdx.x, gridDim.x) * 4 + wave;
+  const int s = sb * 4 + wave;
   if (s >= V3) return;
-  const int lane = cb * 64 + (threadIdx.x & 63);
-  const size_t vo = (size_t)ts * V3 * 3 * K;
+  const int l64 = threadIdx.x & 63;
+  const int lane = cb * 64 + l64;
+  // FP32 filter buffers are block-major: [ts][cb][3*V3][64]
+  const size_t vo = ((size_t)ts * (K / 64) + cb) * (size_t)V3 * 3 * 64;
   const float2* Ut = U32 + (size_t)ts * V3 * 27;
   float2 acc[3], ctr[3];
-  hop<float2>(Ut, in + vo, s, L, K, lane, acc, ctr);
+  hop<float2>(Ut, in + vo, s, L, 64, l64, acc, ctr);
   const float4 cf = coef[(size_t)ts * K + lane];
   const float2 ca = cab[ts];  // (c_t, a_t - 6 c_t)
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const size_t idx = vo + (size_t)(3 * s + c) * K + lane;
+    const size_t idx = vo + (size_t)(3 * s + c) * 64 + l64;
     float2 bv;
     bv.x = ca.x * acc[c].x + ca.y * ctr[c].x;
     bv.y = ca.x * acc[c].y + ca.y * ctr[c].y;
-    const float2 rv = r[idx];
+    const float2 rv = ld_nt(r + idx);
     float2 o;
     o.x = cf.x * bv.x + cf.y * rv.x;
     o.y = cf.x * bv.y + cf.y * rv.y;
     if (use_old) {
-      const float2 ov = old[idx];
+      const float2 ov = ld_nt(old + idx);
       o.x += cf.z * ov.x;
       o.y += cf.z * ov.y;
     }
-    out[idx] = o;
+    st_nt(out + idx, o);
   }
 }
 
 // FP64 Laplacian on a block: out = H v - 6 v (= D v) for all columns.
 __global__ __launch_bounds__(256) void k_lap_block64(const double2* __restrict__ U64,
                                                       const double2* __restrict__ v,
-                                                      double2* __restrict__ out, int L, int K,
+                                                      double2* __restrict__ out, int L, int K, int nts,
                                                       const int* __restrict__ first_blk) {
-  const int ts = blockIdx.z;
-  const int cb = blockIdx.y;
-  if (first_blk && cb < first_blk[ts]) return;
   const int V3 = L * L * L;
+  int ts, cb, sb;
+  unit_map((V3 + 3) / 4, K / 64, nts, ts, cb, sb);
+  if (first_blk && cb < first_blk[ts]) return;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int s = xcd_remap(blockIdx.x, gridDim.x) * 4 + wave;
+  const int s = sb * 4 + wave;
   if (s >= V3) return;
   const int lane = cb * 64 + (threadIdx.x & 63);
   const size_t vo = (size_t)ts * V3 * 3 * K;
@@ -151,24 +181,8 @@ __global__ __launch_bounds__(256) void k_lap_block64(const double2* __restrict__
     double2 o;
     o.x = acc[c].x - 6.0 * ctr[c].x;
     o.y = acc[c].y - 6.0 * ctr[c].y;
-    out[vo + (size_t)(3 * s + c) * K + lane] = o;
+    st_nt(out + vo + (size_t)(3 * s + c) * K + lane, o);
   }
-}
-
-// Y = X + F (FP64 + FP32 correction) on the active columns.
-__global__ void k_combine(double2* __restrict__ X, const float2* __restrict__ F, size_t per_ts, int K,
-                          const int* __restrict__ first_blk) {
-  const int ts = blockIdx.y;
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= per_ts) return;
-  const int col = (int)(i % K);
-  if (col < first_blk[ts] * 64) return;
-  const size_t idx = (size_t)ts * per_ts + i;
-  const float2 f = F[idx];
-  double2 x = X[idx];
-  x.x += (double)f.x;
-  x.y += (double)f.y;
-  X[idx] = x;
 }
 
 // R = AX - X diag(lam); r32 = c_t * R (B-space residual, FP32); nrm2[ts][col] += |R|^2,
@@ -194,7 +208,8 @@ __global__ __launch_bounds__(256) void k_residual(const double2* __restrict__ X,
     const double2 a = AX[idx], x = X[idx];
     const double rx = a.x - l * x.x, ry = a.y - l * x.y;
     acc += rx * rx + ry * ry;
-    r32[idx] = make_float2((float)(c * rx), (float)(c * ry));
+    r32[(((size_t)ts * (K / 64) + (col >> 6)) * n + row) * 64 + (col & 63)] =
+        make_float2((float)(c * rx), (float)(c * ry));
   }
   __shared__ double part[256];
   part[threadIdx.x] = acc;
@@ -253,27 +268,20 @@ __global__ void k_random_block(double2* __restrict__ X, size_t total, unsigned l
 // ---------------------------------------------------------------------------
 static inline dim3 site_grid(int L, int K, int nts) {
   const int V3 = L * L * L;
-  return dim3((V3 + 3) / 4, K / 64, nts);
+  return dim3((unsigned)((V3 + 3) / 4) * (unsigned)(K / 64) * (unsigned)nts);
 }
 
 hipError_t launch_filter_step32(const float2* U32, const float2* in, const float2* r, const float2* old,
                                 float2* out, const float4* coef, const float2* cab, const int* first_blk,
                                 int L, int K, int nts, hipStream_t st) {
-  k_filter_step32<<<site_grid(L, K, nts), 256, 0, st>>>(U32, in, r, old, out, coef, cab, first_blk, L, K,
+  k_filter_step32<<<site_grid(L, K, nts), 256, 0, st>>>(U32, in, r, old, out, coef, cab, first_blk, L, K, nts,
                                                         old != nullptr);
   return hipGetLastError();
 }
 
 hipError_t launch_lap_block64(const double2* U64, const double2* v, double2* out, int L, int K, int nts,
                               const int* first_blk, hipStream_t st) {
-  k_lap_block64<<<site_grid(L, K, nts), 256, 0, st>>>(U64, v, out, L, K, first_blk);
-  return hipGetLastError();
-}
-
-hipError_t launch_combine(double2* X, const float2* F, size_t per_ts, int K, int nts, const int* first_blk,
-                          hipStream_t st) {
-  dim3 g((unsigned)((per_ts + 255) / 256), nts);
-  k_combine<<<g, 256, 0, st>>>(X, F, per_ts, K, first_blk);
+  k_lap_block64<<<site_grid(L, K, nts), 256, 0, st>>>(U64, v, out, L, K, nts, first_blk);
   return hipGetLastError();
 }
 
