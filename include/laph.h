@@ -161,7 +161,9 @@ typedef struct laph_batch_opts {
   double tol;        /* lock when |D v - lambda v| <= tol (v unit norm) */
   int max_iter;      /* filter / Rayleigh-Ritz cycles (0 = 60) */
   unsigned long long seed;
-  int warm_start;    /* 1: start from the block set by laph_batch_set_start */
+  int warm_start;    /* 1: start from the block proposed by laph_net_propose, when one was */
+  int filter_passes; /* applications of T_DEG(B) per Rayleigh-Ritz cycle once the window has
+                        settled: 1 or 2 (0 = 2) */
 } laph_batch_opts;
 
 typedef struct laph_batch_stats {

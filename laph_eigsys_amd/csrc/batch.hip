@@ -48,6 +48,8 @@
 
 #include "batch_internal.h"
 
+#include <cstdio>
+#include <cstdlib>
 #include <thread>
 
 namespace laph {
@@ -73,6 +75,7 @@ static int batch_alloc(laph_batch* b) {
   LAPH_HIP(hipMalloc(&b->cts, b->nts * sizeof(double)));
   LAPH_HIP(hipMalloc(&b->gam, nk * sizeof(double)));
   LAPH_HIP(hipMalloc(&b->coef, (size_t)std::max(1, b->deg - 1) * nk * sizeof(float4)));
+  LAPH_HIP(hipMalloc(&b->coef2, (size_t)b->deg * nk * sizeof(float4)));
   LAPH_HIP(hipMalloc(&b->cab, b->nts * sizeof(float2)));
   LAPH_HIP(hipMalloc(&b->first_blk, b->nts * sizeof(int)));
   LAPH_HIP(hipMalloc(&b->lock_blk, b->nts * sizeof(int)));
@@ -99,6 +102,7 @@ static void batch_free(laph_batch* b) {
   for (auto& ln : b->lanes) {
     if (ln.st) (void)hipStreamSynchronize(ln.st);
     if (ln.graph) (void)hipGraphExecDestroy(ln.graph);
+    if (ln.graph2) (void)hipGraphExecDestroy(ln.graph2);
     for (auto e : ln.ev)
       if (e) (void)hipEventDestroy(e);
     if (ln.rb) rocblas_destroy_handle(ln.rb);
@@ -106,7 +110,7 @@ static void batch_free(laph_batch* b) {
   }
   void* bufs[] = {b->U64, b->U32, b->dec,  b->Unew, b->B[0], b->B[1],   b->B[2], b->R32,     b->P[0],
                   b->P[1], b->P[2], b->G,  b->G2,   b->P2,   b->mu,     b->E,    b->info,    b->nrm2,
-                  b->lam,  b->cts,  b->gam, b->coef, b->cab, b->first_blk, b->lock_blk, b->cn};
+                  b->lam,  b->cts,  b->gam, b->coef, b->cab, b->first_blk, b->lock_blk, b->cn, b->coef2};
   for (void* q : bufs)
     if (q) (void)hipFree(q);
   if (b->st) (void)hipStreamDestroy(b->st);
@@ -330,7 +334,7 @@ static int upload_coefficients(const View& v, const std::vector<double>& lc) {
   laph_lane* ln = v.ln;
   const int K = b->K, nt = ln->nt, m = b->deg;
   const double lamL = b->p.lambda_l;
-  std::vector<float4> co((size_t)(m - 1) * nt * K);
+  std::vector<float4> co((size_t)(m - 1) * nt * K), co2((size_t)m * nt * K);
   std::vector<double> ga((size_t)nt * K);
   std::vector<float2> cab(nt);
   std::vector<int> fb(nt);
@@ -360,12 +364,21 @@ static int upload_coefficients(const View& v, const std::vector<double>& lc) {
         }
         co[((size_t)(st - 1) * nt + t) * K + j] = q;
       }
+      // plain pass h_k = T_k(B) f / s_k: h_1 = B f / s_1,
+      // h_{k+1} = (2 s_k/s_{k+1}) B h_k - (s_{k-1}/s_{k+1}) h_{k-1}, h_0 = f
+      co2[((size_t)0 * nt + t) * K + j] = make_float4((float)(1.0 / s[1]), 0.f, 0.f, 0.f);
+      for (int st = 2; st <= m; ++st)
+        co2[((size_t)(st - 1) * nt + t) * K + j] =
+            make_float4((float)(2.0 * s[st - 1] / s[st]), 0.f, (float)(-s[st - 2] / s[st]), 0.f);
       ga[(size_t)t * K + j] = tau[m] / s[m];
     }
   }
   // coefficient table of the lane: [step][lane timeslice][K] at offset step*nts*K + t0*K
   for (int st = 1; st < m; ++st)
     LAPH_HIP(hipMemcpyAsync(b->coef + (size_t)(st - 1) * b->nts * K + v.nk(), co.data() + (size_t)(st - 1) * nt * K,
+                            (size_t)nt * K * sizeof(float4), hipMemcpyHostToDevice, ln->st));
+  for (int st = 1; st <= m; ++st)
+    LAPH_HIP(hipMemcpyAsync(b->coef2 + (size_t)(st - 1) * b->nts * K + v.nk(), co2.data() + (size_t)(st - 1) * nt * K,
                             (size_t)nt * K * sizeof(float4), hipMemcpyHostToDevice, ln->st));
   LAPH_HIP(hipMemcpyAsync(b->gam + v.nk(), ga.data(), ga.size() * sizeof(double), hipMemcpyHostToDevice, ln->st));
   LAPH_HIP(hipMemcpyAsync(b->cab + ln->t0, cab.data(), cab.size() * sizeof(float2), hipMemcpyHostToDevice, ln->st));
@@ -390,26 +403,72 @@ static int enqueue_filter(const View& v) {
   return LAPH_OK;
 }
 
-// Replays the captured filter of the lane (captured on first use).
-static int run_filter(const View& v, float2** fm) {
-  laph_lane* ln = v.ln;
-  if (!ln->graph) {
-    hipGraph_t g;
-    LAPH_HIP(hipStreamBeginCapture(ln->st, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_filter(v);
-    hipError_t e = hipStreamEndCapture(ln->st, &g);
-    LAPH_TRY(rc);
-    LAPH_HIP(e);
-    LAPH_HIP(hipGraphInstantiate(&ln->graph, g, nullptr, nullptr, 0));
-    LAPH_HIP(hipGraphDestroy(g));
+// Buffers of the plain pass: f_m stays where the correction pass left it
+// (Pf); h_k rotates over the other three buffers (R32 is free by then).
+static void plain_buffers(const View& v, float2** pf, float2* q[3]) {
+  laph_batch* b = v.b;
+  const int m = b->deg;
+  float2* all[4] = {b->R32, b->P[0], b->P[1], b->P[2]};
+  float2* f = b->P[(m - 2) % 3];
+  int k = 0;
+  for (float2* a : all)
+    if (a != f) q[k++] = a + v.vo();
+  *pf = f + v.vo();
+}
+
+// The m plain steps h_1 .. h_m applied to f_m.
+static int enqueue_filter_plain(const View& v) {
+  laph_batch* b = v.b;
+  const int m = b->deg, K = b->K, nt = v.ln->nt, L = b->p.L;
+  const size_t step = (size_t)b->nts * K;
+  float2 *f, *q[3];
+  plain_buffers(v, &f, q);
+  for (int st = 1; st <= m; ++st) {
+    const float2* in = st == 1 ? f : q[(st - 2) % 3];
+    const float2* old = st == 1 ? nullptr : (st == 2 ? f : q[(st - 3) % 3]);
+    float2* out = q[(st - 1) % 3];
+    LAPH_HIP(launch_filter_step32(b->U32 + v.lo(), in, nullptr, old, out, b->coef2 + (st - 1) * step + v.nk(),
+                                  b->cab + v.ln->t0, b->first_blk + v.ln->t0, L, K, nt, v.ln->st));
   }
-  *fm = v.b->P[(v.b->deg - 2) % 3] + v.vo();
-  LAPH_HIP(hipGraphLaunch(ln->graph, ln->st));
   return LAPH_OK;
 }
 
-__global__ void k_combine_gamma(double2* __restrict__ X, const float2* __restrict__ F, const double* __restrict__ gam,
-                                size_t per_ts, int K, const int* __restrict__ first_blk) {
+static int capture(const View& v, hipGraphExec_t* exec, int (*enqueue)(const View&)) {
+  laph_lane* ln = v.ln;
+  hipGraph_t g;
+  LAPH_HIP(hipStreamBeginCapture(ln->st, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue(v);
+  hipError_t e = hipStreamEndCapture(ln->st, &g);
+  LAPH_TRY(rc);
+  LAPH_HIP(e);
+  LAPH_HIP(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+  LAPH_HIP(hipGraphDestroy(g));
+  return LAPH_OK;
+}
+
+// Replays the captured correction pass of the lane (captured on first use);
+// with passes == 2 also the plain pass.  *fm <- f_m, *hm <- h_m (or null).
+static int run_filter(const View& v, int passes, float2** fm, float2** hm) {
+  laph_lane* ln = v.ln;
+  if (!ln->graph) LAPH_TRY(capture(v, &ln->graph, enqueue_filter));
+  *fm = v.b->P[(v.b->deg - 2) % 3] + v.vo();
+  *hm = nullptr;
+  LAPH_HIP(hipGraphLaunch(ln->graph, ln->st));
+  if (passes >= 2) {
+    if (!ln->graph2) LAPH_TRY(capture(v, &ln->graph2, enqueue_filter_plain));
+    float2 *f, *q[3];
+    plain_buffers(v, &f, q);
+    *hm = q[(v.b->deg - 1) % 3];
+    LAPH_HIP(hipGraphLaunch(ln->graph2, ln->st));
+  }
+  return LAPH_OK;
+}
+
+// One pass:  y = gamma x + f_m                 (= T(B) x / s_m)
+// Two passes: y = gamma^2 x + gamma f_m + h_m    (= T(B)^2 x / s_m^2)
+__global__ void k_combine_gamma(double2* __restrict__ X, const float2* __restrict__ F, const float2* __restrict__ Hm,
+                                const double* __restrict__ gam, size_t per_ts, int K,
+                                const int* __restrict__ first_blk) {
   const int ts = blockIdx.y;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= per_ts) return;
@@ -418,9 +477,15 @@ __global__ void k_combine_gamma(double2* __restrict__ X, const float2* __restric
   const size_t idx = (size_t)ts * per_ts + i;
   const double g = gam[(size_t)ts * K + col];
   const size_t row = i / K;  // F is block-major: [ts][col/64][row][64]
-  const float2 f = F[(size_t)ts * per_ts + ((size_t)(col >> 6) * (per_ts / K) + row) * 64 + (col & 63)];
+  const size_t fi = (size_t)ts * per_ts + ((size_t)(col >> 6) * (per_ts / K) + row) * 64 + (col & 63);
+  const float2 f = F[fi];
   const double2 x = X[idx];
-  X[idx] = make_double2(g * x.x + (double)f.x, g * x.y + (double)f.y);
+  if (Hm) {
+    const float2 h = Hm[fi];
+    X[idx] = make_double2(g * (g * x.x + (double)f.x) + (double)h.x, g * (g * x.y + (double)f.y) + (double)h.y);
+  } else {
+    X[idx] = make_double2(g * x.x + (double)f.x, g * x.y + (double)f.y);
+  }
 }
 
 // Solve loop of one lane (runs on its own host thread).
@@ -448,6 +513,8 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
   const int max_iter = b->o.max_iter > 0 ? b->o.max_iter : 60;
   int it = 0;
   for (it = 0; it < max_iter && active > 0; ++it) {
+    // one filter pass while the window settles, then T_m(B)^2 per cycle
+    const int passes = it < 2 ? 1 : std::max(1, b->o.filter_passes > 0 ? std::min(b->o.filter_passes, 2) : 2);
     LAPH_TRY(upload_coefficients(v, lc));
     for (int t = 0; t < nt; ++t) {
       const int nl = b->nlock[ln->t0 + t];
@@ -461,14 +528,19 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
         const int touches = s == 1 ? 2 : (s == 2 ? 3 : 4);
         ln->fbytes += (double)b->V3 * ((double)cols * touches * 24.0 + (cols / 64) * 216.0);
       }
+      if (passes == 2) {  // plain pass: step 1 in/out, later steps in/old/out
+        ln->sv32 += (long long)b->deg * cols;
+        for (int s = 1; s <= b->deg; ++s)
+          ln->fbytes += (double)b->V3 * ((double)cols * (s == 1 ? 2 : 3) * 24.0 + (cols / 64) * 216.0);
+      }
     }
-    ln->flaunch += b->deg - 1;
-    float2* fm = nullptr;
+    ln->flaunch += (b->deg - 1) + (passes == 2 ? b->deg : 0);
+    float2 *fm = nullptr, *hm = nullptr;
     LAPH_HIP(hipEventRecord(ln->ev[0], ln->st));
-    LAPH_TRY(run_filter(v, &fm));
+    LAPH_TRY(run_filter(v, passes, &fm, &hm));
     LAPH_HIP(hipEventRecord(ln->ev[1], ln->st));
     dim3 g((unsigned)((b->per_ts + 255) / 256), nt);
-    k_combine_gamma<<<g, 256, 0, ln->st>>>(v.X(), fm, b->gam + v.nk(), b->per_ts, K, b->first_blk + ln->t0);
+    k_combine_gamma<<<g, 256, 0, ln->st>>>(v.X(), fm, hm, b->gam + v.nk(), b->per_ts, K, b->first_blk + ln->t0);
     LAPH_HIP(hipGetLastError());
     const Act q{ln->lock, K - ln->lock};
     LAPH_TRY(project_locked(v, q));
@@ -477,6 +549,13 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
     LAPH_TRY(orthonormalise_and_rr(v, q, it < 2));
     LAPH_HIP(hipEventRecord(ln->ev[2], ln->st));
     LAPH_TRY(residuals_and_window(v, lc, &active));
+    if (b->trace && ln->t0 == 0) {  // per-cycle trace of the first timeslice
+      const double* mu = &b->h_mu[0];
+      const double* rs = &b->h_res[0];
+      std::fprintf(stderr, "[laph] cycle %d: window %.4f  mu[nev-1] %.5f  mu[K-1] %.4f  res[0] %.2e  res[nev-1] %.2e  "
+                   "locked %d  lock %d\n", it, lc[0], mu[b->nev - 1], mu[K - 1], std::sqrt(rs[0]),
+                   std::sqrt(rs[b->nev - 1]), b->nlock[0], ln->lock);
+    }
     float a = 0, c = 0;
     LAPH_HIP(hipEventElapsedTime(&a, ln->ev[0], ln->ev[1]));
     LAPH_HIP(hipEventElapsedTime(&c, ln->ev[1], ln->ev[2]));
@@ -573,6 +652,7 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     b->nev = o->nev;
     b->deg = p->deg;
     b->per_ts = (size_t)n * K;
+    b->trace = std::getenv("LAPH_TRACE") != nullptr;
     b->nlock.assign(b->nts, 0);
     b->h_mu.assign((size_t)b->nts * K, 0.0);
     b->h_res.assign((size_t)b->nts * K, 0.0);
@@ -649,10 +729,10 @@ int laph_batch_time_filter(laph_batch* b, int reps, double* ms_per_launch, doubl
     // every column of every timeslice of lane 0 active
     std::vector<int> zero(nt, 0);
     LAPH_HIP(hipMemcpyAsync(b->first_blk + ln->t0, zero.data(), nt * sizeof(int), hipMemcpyHostToDevice, ln->st));
-    float2* fm = nullptr;
-    LAPH_TRY(laph::run_filter(v, &fm));  // warm-up (and capture if needed)
+    float2 *fm = nullptr, *hm = nullptr;
+    LAPH_TRY(laph::run_filter(v, 1, &fm, &hm));  // warm-up (and capture if needed)
     LAPH_HIP(hipEventRecord(ln->ev[0], ln->st));
-    for (int r = 0; r < reps; ++r) LAPH_TRY(laph::run_filter(v, &fm));
+    for (int r = 0; r < reps; ++r) LAPH_TRY(laph::run_filter(v, 1, &fm, &hm));
     LAPH_HIP(hipEventRecord(ln->ev[1], ln->st));
     LAPH_HIP(hipEventSynchronize(ln->ev[1]));
     float ms = 0;

@@ -26,7 +26,8 @@ struct laph_lane {
   rocblas_handle rb = nullptr;
   int iX = 0, iAX = 1, iT = 2;  // roles of the three FP64 block buffers
   int lock = 0;                 // hard-locked columns, common to the lane
-  hipGraphExec_t graph = nullptr;
+  hipGraphExec_t graph = nullptr;   // correction-form filter pass
+  hipGraphExec_t graph2 = nullptr;  // plain filter pass (second application)
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   // per-solve statistics
   int cycles = 0, rc = 0;
@@ -64,6 +65,7 @@ struct laph_batch {
   double* cts = nullptr;     // [nts]
   double* gam = nullptr;     // [nts][K]
   float4* coef = nullptr;    // [deg-1][nts][K]
+  float4* coef2 = nullptr;   // [deg][nts][K] plain pass
   float2* cab = nullptr;     // [nts]
   int* first_blk = nullptr;  // [nts] first filtered 64-column block
   int* lock_blk = nullptr;   // [nts] first active 64-column block of the dense steps
@@ -72,6 +74,7 @@ struct laph_batch {
   std::vector<double> h_mu, h_res;
   std::vector<laph_lane> lanes;
   bool solved = false;
+  bool trace = false;       // LAPH_TRACE set: per-cycle trace of timeslice 0 on stderr
   bool have_start = false;  // B[0] holds a proposed start block (laph_net_propose)
 };
 

@@ -8,7 +8,8 @@ namespace laph {
 
 // out = a1 (B in) + a2 r + b old per column, B = c_t (H - 6) + a_t;
 // coef[ts*K + col] = (a1, a2, b, -), cab[ts] = (c_t, a_t - 6 c_t); column
-// blocks below first_blk[ts] are skipped.  old may be null.
+// blocks below first_blk[ts] are skipped.  old and r may be null (plain
+// recurrence without the source term).
 hipError_t launch_filter_step32(const float2* U32, const float2* in, const float2* r, const float2* old,
                                 float2* out, const float4* coef, const float2* cab, const int* first_blk,
                                 int L, int K, int nts, hipStream_t st);

@@ -113,9 +113,11 @@ __device__ __forceinline__ void hop(const C* __restrict__ U, const C* __restrict
   }
 }
 
-// One correction-form Chebyshev step in FP32 for all timeslices:
+// One Chebyshev step in FP32 for all timeslices:
 //   out = a1 * (B in) + a2 * r + b * old,   B = c_t (H - 6) + a_t
-// with per-column coefficients (a1, a2, b) for this step.
+// with per-column coefficients (a1, a2, b) for this step; HAS_R = false is
+// the plain recurrence (no source term, one vector stream less).
+template <bool HAS_R>
 __global__ __launch_bounds__(256) void k_filter_step32(const float2* __restrict__ U32,
                                                         const float2* __restrict__ in,
                                                         const float2* __restrict__ r,
@@ -147,10 +149,14 @@ __global__ __launch_bounds__(256) void k_filter_step32(const float2* __restrict_
     float2 bv;
     bv.x = ca.x * acc[c].x + ca.y * ctr[c].x;
     bv.y = ca.x * acc[c].y + ca.y * ctr[c].y;
-    const float2 rv = ld_nt(r + idx);
     float2 o;
-    o.x = cf.x * bv.x + cf.y * rv.x;
-    o.y = cf.x * bv.y + cf.y * rv.y;
+    o.x = cf.x * bv.x;
+    o.y = cf.x * bv.y;
+    if (HAS_R) {
+      const float2 rv = ld_nt(r + idx);
+      o.x += cf.y * rv.x;
+      o.y += cf.y * rv.y;
+    }
     if (use_old) {
       const float2 ov = ld_nt(old + idx);
       o.x += cf.z * ov.x;
@@ -274,8 +280,12 @@ static inline dim3 site_grid(int L, int K, int nts) {
 hipError_t launch_filter_step32(const float2* U32, const float2* in, const float2* r, const float2* old,
                                 float2* out, const float4* coef, const float2* cab, const int* first_blk,
                                 int L, int K, int nts, hipStream_t st) {
-  k_filter_step32<<<site_grid(L, K, nts), 256, 0, st>>>(U32, in, r, old, out, coef, cab, first_blk, L, K, nts,
-                                                        old != nullptr);
+  if (r)
+    k_filter_step32<true><<<site_grid(L, K, nts), 256, 0, st>>>(U32, in, r, old, out, coef, cab, first_blk, L, K, nts,
+                                                              old != nullptr);
+  else
+    k_filter_step32<false><<<site_grid(L, K, nts), 256, 0, st>>>(U32, in, r, old, out, coef, cab, first_blk, L, K,
+                                                               nts, old != nullptr);
   return hipGetLastError();
 }
 
