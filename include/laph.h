@@ -177,7 +177,9 @@ typedef struct laph_batch_stats {
   double seconds_dense;    /* device time in Gram / CholQR / RR / rotations */
   long long filter_launches; /* FP32 filter-step kernel launches */
   double filter_bytes;     /* algorithmic bytes of those launches (vectors read/written
-                              once, links once per 64-vector block) */
+                              once, links once per vector block) */
+  int filter_block;        /* vectors per block of the FP32 filter (64 or 128, chosen by
+                              timing both on the first solve unless LAPH_BW fixes it) */
 } laph_batch_stats;
 
 int laph_batch_create(laph_batch** out, const laph_params* params, const laph_batch_opts* opts);

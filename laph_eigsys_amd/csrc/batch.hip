@@ -80,12 +80,30 @@ static int batch_alloc(laph_batch* b) {
   LAPH_HIP(hipMalloc(&b->first_blk, b->nts * sizeof(int)));
   LAPH_HIP(hipMalloc(&b->lock_blk, b->nts * sizeof(int)));
   LAPH_HIP(hipMalloc(&b->cn, nk * sizeof(double)));
-  const int nlanes = b->nts >= 2 ? 2 : 1;
+  LAPH_HIP(hipMalloc(&b->jsweeps, b->nts * sizeof(rocblas_int)));
+  // defined contents for the width autotuner's first replays
+  LAPH_HIP(hipMemset(b->coef, 0, (size_t)std::max(1, b->deg - 1) * nk * sizeof(float4)));
+  LAPH_HIP(hipMemset(b->coef2, 0, (size_t)b->deg * nk * sizeof(float4)));
+  LAPH_HIP(hipMemset(b->cab, 0, b->nts * sizeof(float2)));
+  LAPH_HIP(hipMemset(b->first_blk, 0, b->nts * sizeof(int)));
+  LAPH_HIP(hipMemset(b->R32, 0, nb * sizeof(float2)));
+  for (int i = 0; i < 3; ++i) LAPH_HIP(hipMemset(b->P[i], 0, nb * sizeof(float2)));
+  if (const char* e = std::getenv("LAPH_BW")) {
+    const int w = std::atoi(e);
+    LAPH_CHECK(w == 64 || (w == 128 && b->K % 128 == 0), LAPH_ERR_INVALID, "LAPH_BW=%s: 64, or 128 with K %% 128 == 0",
+               e);
+    b->bw = w;
+  } else if (b->K % 128 != 0) {
+    b->bw = 64;
+  }
+  int nlanes = 2;
+  if (const char* e = std::getenv("LAPH_LANES")) nlanes = std::max(1, std::min(4, std::atoi(e)));
+  nlanes = std::min(nlanes, b->nts);
   b->lanes.resize(nlanes);
   for (int h = 0; h < nlanes; ++h) {
     laph_lane& ln = b->lanes[h];
-    ln.t0 = h * (b->nts / nlanes);
-    ln.nt = (h == nlanes - 1) ? b->nts - ln.t0 : b->nts / nlanes;
+    ln.t0 = (int)((long long)h * b->nts / nlanes);
+    ln.nt = (int)((long long)(h + 1) * b->nts / nlanes) - ln.t0;
     LAPH_HIP(hipStreamCreateWithFlags(&ln.st, hipStreamNonBlocking));
     LAPH_CHECK(rocblas_create_handle(&ln.rb) == rocblas_status_success &&
                    rocblas_set_stream(ln.rb, ln.st) == rocblas_status_success,
@@ -110,7 +128,7 @@ static void batch_free(laph_batch* b) {
   }
   void* bufs[] = {b->U64, b->U32, b->dec,  b->Unew, b->B[0], b->B[1],   b->B[2], b->R32,     b->P[0],
                   b->P[1], b->P[2], b->G,  b->G2,   b->P2,   b->mu,     b->E,    b->info,    b->nrm2,
-                  b->lam,  b->cts,  b->gam, b->coef, b->cab, b->first_blk, b->lock_blk, b->cn, b->coef2};
+                  b->lam,  b->cts,  b->gam, b->coef, b->cab, b->first_blk, b->lock_blk, b->cn, b->coef2, b->jsweeps};
   for (void* q : bufs)
     if (q) (void)hipFree(q);
   if (b->st) (void)hipStreamDestroy(b->st);
@@ -241,8 +259,17 @@ static int rayleigh_ritz(const View& v, Act q, bool orthonormal, bool* chol_fail
     LAPH_RB(rocblas_ztrsm_strided_batched(rb, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                                           rocblas_diagonal_non_unit, q.a, q.a, &one, zc(G2), K, kk, zc(G), K, kk, nt));
   }
-  LAPH_RB(rocsolver_zheevd_strided_batched(rb, rocblas_evect_original, rocblas_fill_upper, q.a, zc(G), K, kk,
-                                           b->mu + v.nk() + q.l, K, b->E + v.nk(), K, info, nt));
+  if (b->eig_mode == 1) {
+    LAPH_RB(rocsolver_zheevdj_strided_batched(rb, rocblas_evect_original, rocblas_fill_upper, q.a, zc(G), K, kk,
+                                              b->mu + v.nk() + q.l, K, info, nt));
+  } else if (b->eig_mode == 2) {
+    LAPH_RB(rocsolver_zheevj_strided_batched(rb, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_upper,
+                                             q.a, zc(G), K, kk, 0.0, b->E + v.nk(), 100, b->jsweeps + v.ln->t0,
+                                             b->mu + v.nk() + q.l, K, info, nt));
+  } else {
+    LAPH_RB(rocsolver_zheevd_strided_batched(rb, rocblas_evect_original, rocblas_fill_upper, q.a, zc(G), K, kk,
+                                             b->mu + v.nk() + q.l, K, b->E + v.nk(), K, info, nt));
+  }
   LAPH_TRY(dense_sync_info(v, "Rayleigh-Ritz (zheevd)"));
   if (!orthonormal)
     LAPH_RB(rocblas_ztrsm_strided_batched(rb, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
@@ -301,7 +328,7 @@ static int residuals_and_window(const View& v, std::vector<double>& lc, int* n_a
   LAPH_HIP(hipMemcpyAsync(b->lam + v.nk(), hl.data(), hl.size() * sizeof(double), hipMemcpyHostToDevice, ln->st));
   LAPH_HIP(hipMemcpyAsync(b->cts + ln->t0, hc.data(), hc.size() * sizeof(double), hipMemcpyHostToDevice, ln->st));
   LAPH_HIP(launch_residual(v.X(), v.AX(), b->lam + v.nk(), b->cts + ln->t0, b->R32 + v.vo(), b->nrm2 + v.nk(), b->n,
-                           K, nt, ln->lock, ln->st));
+                           K, nt, ln->lock, b->bw, ln->st));
   LAPH_HIP(hipMemcpyAsync(b->h_res.data() + v.nk(), b->nrm2 + v.nk(), (size_t)nt * K * sizeof(double),
                           hipMemcpyDeviceToHost, ln->st));
   LAPH_HIP(hipStreamSynchronize(ln->st));
@@ -398,7 +425,7 @@ static int enqueue_filter(const View& v) {
     const float2* old = st >= 3 ? b->P[(st - 3) % 3] + vo : nullptr;
     float2* out = b->P[(st - 1) % 3] + vo;
     LAPH_HIP(launch_filter_step32(b->U32 + v.lo(), in, R, old, out, b->coef + (st - 1) * step + v.nk(),
-                                  b->cab + v.ln->t0, b->first_blk + v.ln->t0, L, K, nt, v.ln->st));
+                                  b->cab + v.ln->t0, b->first_blk + v.ln->t0, L, K, nt, b->bw, v.ln->st));
   }
   return LAPH_OK;
 }
@@ -428,7 +455,7 @@ static int enqueue_filter_plain(const View& v) {
     const float2* old = st == 1 ? nullptr : (st == 2 ? f : q[(st - 3) % 3]);
     float2* out = q[(st - 1) % 3];
     LAPH_HIP(launch_filter_step32(b->U32 + v.lo(), in, nullptr, old, out, b->coef2 + (st - 1) * step + v.nk(),
-                                  b->cab + v.ln->t0, b->first_blk + v.ln->t0, L, K, nt, v.ln->st));
+                                  b->cab + v.ln->t0, b->first_blk + v.ln->t0, L, K, nt, b->bw, v.ln->st));
   }
   return LAPH_OK;
 }
@@ -468,7 +495,7 @@ static int run_filter(const View& v, int passes, float2** fm, float2** hm) {
 // Two passes: y = gamma^2 x + gamma f_m + h_m    (= T(B)^2 x / s_m^2)
 __global__ void k_combine_gamma(double2* __restrict__ X, const float2* __restrict__ F, const float2* __restrict__ Hm,
                                 const double* __restrict__ gam, size_t per_ts, int K,
-                                const int* __restrict__ first_blk) {
+                                const int* __restrict__ first_blk, int bw) {
   const int ts = blockIdx.y;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= per_ts) return;
@@ -476,8 +503,8 @@ __global__ void k_combine_gamma(double2* __restrict__ X, const float2* __restric
   if (col < first_blk[ts] * 64) return;
   const size_t idx = (size_t)ts * per_ts + i;
   const double g = gam[(size_t)ts * K + col];
-  const size_t row = i / K;  // F is block-major: [ts][col/64][row][64]
-  const size_t fi = (size_t)ts * per_ts + ((size_t)(col >> 6) * (per_ts / K) + row) * 64 + (col & 63);
+  const size_t row = i / K;  // F is block-major: [ts][col/bw][row][bw]
+  const size_t fi = (size_t)ts * per_ts + ((size_t)(col / bw) * (per_ts / K) + row) * bw + (col % bw);
   const float2 f = F[fi];
   const double2 x = X[idx];
   if (Hm) {
@@ -520,18 +547,19 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
       const int nl = b->nlock[ln->t0 + t];
       if (nl >= b->nev) continue;
       const int cols = K - (nl / 64) * 64;
+      const int nblk = (cols + b->bw - 1) / b->bw;  // blocks the kernel visits (links once each)
       ln->sv32 += (long long)(b->deg - 1) * cols;
       // algorithmic bytes: step 1 reads r and writes f_2 (2 vector touches),
       // step 2 adds r (3), later steps add f_{k-1} (4); a vector touch is
       // 3 colours x 8 B per site; links 27 x 8 B per site per 64-vector block
       for (int s = 1; s < b->deg; ++s) {
         const int touches = s == 1 ? 2 : (s == 2 ? 3 : 4);
-        ln->fbytes += (double)b->V3 * ((double)cols * touches * 24.0 + (cols / 64) * 216.0);
+        ln->fbytes += (double)b->V3 * ((double)cols * touches * 24.0 + nblk * 216.0);
       }
       if (passes == 2) {  // plain pass: step 1 in/out, later steps in/old/out
         ln->sv32 += (long long)b->deg * cols;
         for (int s = 1; s <= b->deg; ++s)
-          ln->fbytes += (double)b->V3 * ((double)cols * (s == 1 ? 2 : 3) * 24.0 + (cols / 64) * 216.0);
+          ln->fbytes += (double)b->V3 * ((double)cols * (s == 1 ? 2 : 3) * 24.0 + nblk * 216.0);
       }
     }
     ln->flaunch += (b->deg - 1) + (passes == 2 ? b->deg : 0);
@@ -540,7 +568,8 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
     LAPH_TRY(run_filter(v, passes, &fm, &hm));
     LAPH_HIP(hipEventRecord(ln->ev[1], ln->st));
     dim3 g((unsigned)((b->per_ts + 255) / 256), nt);
-    k_combine_gamma<<<g, 256, 0, ln->st>>>(v.X(), fm, hm, b->gam + v.nk(), b->per_ts, K, b->first_blk + ln->t0);
+    k_combine_gamma<<<g, 256, 0, ln->st>>>(v.X(), fm, hm, b->gam + v.nk(), b->per_ts, K, b->first_blk + ln->t0,
+                                           b->bw);
     LAPH_HIP(hipGetLastError());
     const Act q{ln->lock, K - ln->lock};
     LAPH_TRY(project_locked(v, q));
@@ -572,8 +601,50 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
   return LAPH_OK;
 }
 
+// Device time of `reps` replays of lane 0's correction pass at the current
+// block width (graph recaptured for that width).
+static int time_correction_pass(laph_batch* b, int reps, float* ms) {
+  laph_lane* ln = &b->lanes[0];
+  const View v{b, ln};
+  if (ln->graph) LAPH_HIP(hipGraphExecDestroy(ln->graph));
+  ln->graph = nullptr;
+  LAPH_TRY(capture(v, &ln->graph, enqueue_filter));
+  LAPH_HIP(hipGraphLaunch(ln->graph, ln->st));  // warm-up
+  LAPH_HIP(hipEventRecord(ln->ev[0], ln->st));
+  for (int r = 0; r < reps; ++r) LAPH_HIP(hipGraphLaunch(ln->graph, ln->st));
+  LAPH_HIP(hipEventRecord(ln->ev[1], ln->st));
+  LAPH_HIP(hipEventSynchronize(ln->ev[1]));
+  LAPH_HIP(hipEventElapsedTime(ms, ln->ev[0], ln->ev[1]));
+  return LAPH_OK;
+}
+
+// Block width of the FP32 filter buffers: both widths timed on lane 0's
+// timeslices with every column active (the kernels only stream the buffers;
+// their contents do not change the time), the faster kept.
+static int autotune_block_width(laph_batch* b) {
+  laph_lane* ln = &b->lanes[0];
+  LAPH_HIP(hipMemsetAsync(b->first_blk + ln->t0, 0, ln->nt * sizeof(int), ln->st));
+  float best = 0;
+  int wbest = 64;
+  for (int w : {64, 128}) {
+    b->bw = w;
+    float ms = 0;
+    LAPH_TRY(time_correction_pass(b, 3, &ms));
+    if (w == 64 || ms < best) {
+      best = ms;
+      wbest = w;
+    }
+  }
+  b->bw = wbest;
+  LAPH_HIP(hipGraphExecDestroy(ln->graph));  // captured for the last width timed
+  ln->graph = nullptr;
+  if (b->trace) std::fprintf(stderr, "[laph] filter block width %d\n", wbest);
+  return LAPH_OK;
+}
+
 static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
   auto t0 = std::chrono::steady_clock::now();
+  if (b->bw == 0) LAPH_TRY(autotune_block_width(b));
   const bool warm = b->o.warm_start && b->have_start;
   b->have_start = false;
   b->solved = false;
@@ -611,6 +682,7 @@ static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
     s.seconds_dense += ln.ms_dense * 1e-3;
   }
   s.iterations = cycles;
+  s.filter_block = b->bw;
   s.converged = conv;
   s.max_residual = maxres;
   s.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -653,6 +725,7 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     b->deg = p->deg;
     b->per_ts = (size_t)n * K;
     b->trace = std::getenv("LAPH_TRACE") != nullptr;
+    if (const char* e = std::getenv("LAPH_EIG")) b->eig_mode = std::atoi(e);
     b->nlock.assign(b->nts, 0);
     b->h_mu.assign((size_t)b->nts * K, 0.0);
     b->h_res.assign((size_t)b->nts * K, 0.0);
@@ -741,7 +814,7 @@ int laph_batch_time_filter(laph_batch* b, int reps, double* ms_per_launch, doubl
     double bytes = 0;
     for (int s = 1; s < b->deg; ++s) {
       const int touches = s == 1 ? 2 : (s == 2 ? 3 : 4);
-      bytes += (double)nt * b->V3 * ((double)K * touches * 24.0 + (K / 64) * 216.0);
+      bytes += (double)nt * b->V3 * ((double)K * touches * 24.0 + (K / b->bw) * 216.0);
     }
     *ms_per_launch = ms / ((double)reps * steps);
     *bytes_per_launch = bytes / steps;

@@ -74,7 +74,12 @@ struct laph_batch {
   std::vector<double> h_mu, h_res;
   std::vector<laph_lane> lanes;
   bool solved = false;
+  int eig_mode = 0;          // Rayleigh-Ritz eigensolver: 0 zheevd, 1 zheevdj, 2 zheevj (LAPH_EIG)
+  rocblas_int* jsweeps = nullptr;
   bool trace = false;       // LAPH_TRACE set: per-cycle trace of timeslice 0 on stderr
   bool have_start = false;  // B[0] holds a proposed start block (laph_net_propose)
+  // vectors per block of the FP32 filter buffers (64 or 128); 0 until the
+  // first solve times both widths (LAPH_BW=64|128 fixes it)
+  int bw = 0;
 };
 

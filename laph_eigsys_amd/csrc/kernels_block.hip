@@ -2,8 +2,9 @@
 // timeslice are stored site-major with the vectors fastest:
 //     X[ts][(s*3 + colour)*K + col]            (K a multiple of 64)
 // (the layout the strided-batched GEMMs read as a K x n matrix), the FP32
-// filter buffers block-major, one contiguous array per 64-vector block:
-//     F[ts][col/64][(s*3 + colour)*64 + col%64]
+// filter buffers block-major, one contiguous array per block of bw = 64 or
+// 128 vectors (chosen by the batch's autotuner):
+//     F[ts][col/bw][(s*3 + colour)*bw + col%bw]
 // so one 64-lane wavefront holds 64 vectors of ONE site: the links of that
 // site are wave-uniform and are fetched once per wavefront through scalar
 // loads (the link matrix is read once per 64 vectors), while every vector
@@ -55,6 +56,19 @@ __device__ __forceinline__ void st_nt(float2* p, float2 v) {
   w.x = v.x;
   w.y = v.y;
   __builtin_nontemporal_store(w, reinterpret_cast<f2v*>(p));
+}
+typedef float f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld_nt4(const float4* p) {
+  const f4v v = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st_nt4(float4* p, float4 v) {
+  f4v w;
+  w.x = v.x;
+  w.y = v.y;
+  w.z = v.z;
+  w.w = v.w;
+  __builtin_nontemporal_store(w, reinterpret_cast<f4v*>(p));
 }
 __device__ __forceinline__ void st_nt(double2* p, double2 v) {
   d2v w;
@@ -166,6 +180,95 @@ __global__ __launch_bounds__(256) void k_filter_step32(const float2* __restrict_
   }
 }
 
+// 128-vector-block variant: each lane carries two neighbouring vectors as one
+// 16-byte float4 (buffers block-major with 128-vector blocks), halving the
+// wavefronts and the load instructions per vector.
+template <bool HAS_R>
+__global__ __launch_bounds__(256) void k_filter_step32x2(const float2* __restrict__ U32,
+                                                          const float4* __restrict__ in,
+                                                          const float4* __restrict__ r,
+                                                          const float4* __restrict__ old,
+                                                          float4* __restrict__ out,
+                                                          const float4* __restrict__ coef,
+                                                          const float2* __restrict__ cab,
+                                                          const int* __restrict__ first_blk, int L,
+                                                          int K, int nts, int use_old) {
+  const int V3 = L * L * L;
+  int ts, cb, sb;
+  unit_map((V3 + 3) / 4, K / 128, nts, ts, cb, sb);
+  if (2 * cb + 1 < first_blk[ts]) return;  // both 64-vector halves inactive
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int s = sb * 4 + wave;
+  if (s >= V3) return;
+  const int l64 = threadIdx.x & 63;
+  const int col = cb * 128 + 2 * l64;
+  const size_t vo = ((size_t)ts * (K / 128) + cb) * (size_t)V3 * 3 * 64;  // in float4 units
+  const float2* Ut = U32 + (size_t)ts * V3 * 27;
+  const int L2 = L * L;
+  const int x = s / L2, y = (s / L) % L, z = s - (s / L) * L;
+  const int up[3] = {x == L - 1 ? s - (L - 1) * L2 : s + L2, y == L - 1 ? s - (L - 1) * L : s + L,
+                     z == L - 1 ? s - (L - 1) : s + 1};
+  const int dn[3] = {x == 0 ? s + (L - 1) * L2 : s - L2, y == 0 ? s + (L - 1) * L : s - L,
+                     z == 0 ? s + (L - 1) : s - 1};
+  const float4* v = in + vo;
+  float2 acc0[3], acc1[3];
+  float4 ctr[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    acc0[c] = make_float2(0.f, 0.f);
+    acc1[c] = make_float2(0.f, 0.f);
+    ctr[c] = v[(size_t)(3 * s + c) * 64 + l64];
+  }
+#pragma unroll
+  for (int mu = 0; mu < 3; ++mu) {
+    float4 xu[3], xm[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      xu[c] = v[(size_t)(3 * up[mu] + c) * 64 + l64];
+      xm[c] = v[(size_t)(3 * dn[mu] + c) * 64 + l64];
+    }
+    const float2* Uf = Ut + ((size_t)s * 3 + mu) * 9;
+    const float2* Ub = Ut + ((size_t)dn[mu] * 3 + mu) * 9;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+      for (int bb = 0; bb < 3; ++bb) {
+        const float2 uf = Uf[3 * a + bb], ub = Ub[3 * bb + a];
+        acc0[a] = cfma(uf, make_float2(xu[bb].x, xu[bb].y), acc0[a]);
+        acc1[a] = cfma(uf, make_float2(xu[bb].z, xu[bb].w), acc1[a]);
+        acc0[a] = cfmac(ub, make_float2(xm[bb].x, xm[bb].y), acc0[a]);
+        acc1[a] = cfmac(ub, make_float2(xm[bb].z, xm[bb].w), acc1[a]);
+      }
+    }
+  }
+  const float4 cf0 = coef[(size_t)ts * K + col], cf1 = coef[(size_t)ts * K + col + 1];
+  const float2 ca = cab[ts];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t idx = vo + (size_t)(3 * s + c) * 64 + l64;
+    float4 o;
+    o.x = cf0.x * (ca.x * acc0[c].x + ca.y * ctr[c].x);
+    o.y = cf0.x * (ca.x * acc0[c].y + ca.y * ctr[c].y);
+    o.z = cf1.x * (ca.x * acc1[c].x + ca.y * ctr[c].z);
+    o.w = cf1.x * (ca.x * acc1[c].y + ca.y * ctr[c].w);
+    if (HAS_R) {
+      const float4 rv = ld_nt4(r + idx);
+      o.x += cf0.y * rv.x;
+      o.y += cf0.y * rv.y;
+      o.z += cf1.y * rv.z;
+      o.w += cf1.y * rv.w;
+    }
+    if (use_old) {
+      const float4 ov = ld_nt4(old + idx);
+      o.x += cf0.z * ov.x;
+      o.y += cf0.z * ov.y;
+      o.z += cf1.z * ov.z;
+      o.w += cf1.z * ov.w;
+    }
+    st_nt4(out + idx, o);
+  }
+}
+
 // FP64 Laplacian on a block: out = H v - 6 v (= D v) for all columns.
 __global__ __launch_bounds__(256) void k_lap_block64(const double2* __restrict__ U64,
                                                       const double2* __restrict__ v,
@@ -200,7 +303,7 @@ __global__ __launch_bounds__(256) void k_residual(const double2* __restrict__ X,
                                                    const double* __restrict__ cts,
                                                    float2* __restrict__ r32,
                                                    double* __restrict__ nrm2, int n, int K,
-                                                   int rows_per_block, int first_col) {
+                                                   int rows_per_block, int first_col, int bw) {
   const int ts = blockIdx.z;
   const int col = first_col + blockIdx.y * 64 + (threadIdx.x & 63);
   const int r0 = blockIdx.x * rows_per_block + (threadIdx.x >> 6);
@@ -214,7 +317,7 @@ __global__ __launch_bounds__(256) void k_residual(const double2* __restrict__ X,
     const double2 a = AX[idx], x = X[idx];
     const double rx = a.x - l * x.x, ry = a.y - l * x.y;
     acc += rx * rx + ry * ry;
-    r32[(((size_t)ts * (K / 64) + (col >> 6)) * n + row) * 64 + (col & 63)] =
+    r32[(((size_t)ts * (K / bw) + col / bw) * n + row) * bw + col % bw] =
         make_float2((float)(c * rx), (float)(c * ry));
   }
   __shared__ double part[256];
@@ -279,7 +382,22 @@ static inline dim3 site_grid(int L, int K, int nts) {
 
 hipError_t launch_filter_step32(const float2* U32, const float2* in, const float2* r, const float2* old,
                                 float2* out, const float4* coef, const float2* cab, const int* first_blk,
-                                int L, int K, int nts, hipStream_t st) {
+                                int L, int K, int nts, int bw, hipStream_t st) {
+  if (bw == 128) {
+    const int V3 = L * L * L;
+    dim3 g((unsigned)((V3 + 3) / 4) * (unsigned)(K / 128) * (unsigned)nts);
+    const float4* in4 = reinterpret_cast<const float4*>(in);
+    const float4* r4 = reinterpret_cast<const float4*>(r);
+    const float4* o4 = reinterpret_cast<const float4*>(old);
+    float4* out4 = reinterpret_cast<float4*>(out);
+    if (r)
+      k_filter_step32x2<true><<<g, 256, 0, st>>>(U32, in4, r4, o4, out4, coef, cab, first_blk, L, K, nts,
+                                                 old != nullptr);
+    else
+      k_filter_step32x2<false><<<g, 256, 0, st>>>(U32, in4, r4, o4, out4, coef, cab, first_blk, L, K, nts,
+                                                  old != nullptr);
+    return hipGetLastError();
+  }
   if (r)
     k_filter_step32<true><<<site_grid(L, K, nts), 256, 0, st>>>(U32, in, r, old, out, coef, cab, first_blk, L, K, nts,
                                                               old != nullptr);
@@ -296,12 +414,12 @@ hipError_t launch_lap_block64(const double2* U64, const double2* v, double2* out
 }
 
 hipError_t launch_residual(const double2* X, const double2* AX, const double* lam, const double* cts,
-                           float2* r32, double* nrm2, int n, int K, int nts, int first_col, hipStream_t st) {
+                           float2* r32, double* nrm2, int n, int K, int nts, int first_col, int bw, hipStream_t st) {
   const int w = K - first_col;
   k_zero_cols<<<(nts * w + 255) / 256, 256, 0, st>>>(nrm2, nts, K, first_col);
   const int rows = 256;
   dim3 g((n + rows - 1) / rows, w / 64, nts);
-  k_residual<<<g, 256, 0, st>>>(X, AX, lam, cts, r32, nrm2, n, K, rows, first_col);
+  k_residual<<<g, 256, 0, st>>>(X, AX, lam, cts, r32, nrm2, n, K, rows, first_col, bw);
   return hipGetLastError();
 }
 

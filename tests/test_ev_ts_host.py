@@ -1,0 +1,50 @@
+"""Host logic of the per-configuration driver (laph_eigsys_amd/ev_ts.py):
+the parameter file of par_io.cpp:8-133 and the grouping of timeslices into
+batched solves that fit the device memory."""
+from laph_eigsys_amd import ev_ts
+
+
+PARAMS = """# comment line
+config_path = /data/confs
+result_path = ./out
+phase_path = ./old
+#
+LT = 48
+LX = 24
+NEV = 120
+alpha_1 = 0.62
+alpha_2 = 0.62
+iter = 3
+LAM_L = 12.0
+LAM_C = 0.9
+DEG = 12
+OMP_THRDS = 2
+"""
+
+
+def test_parameter_file(tmp_path):
+    f = tmp_path / "parameters.txt"
+    f.write_text(PARAMS)
+    p = ev_ts.IO().set_values(str(f))
+    assert p.get_int("LT") == 48 and p.get_int("LX") == 24 and p.get_int("LZ") == 24
+    assert p.get_int("V3") == 24 ** 3 and p.get_int("MAT_ENTRIES") == 3 * 24 ** 3
+    assert p.get_int("V_TS") == 72 * 24 ** 3 and p.get_int("V4_LIME") == 48 * 72 * 24 ** 3
+    assert p.get_int("NEV") == 120 and p.get_int("DEG") == 12 and p.get_int("iter") == 3
+    assert p.get_float("alpha_1") == 0.62 and p.get_float("LAM_C") == 0.9
+    assert p.get_path("conf") == "/data/confs" and p.get_path("res") == "./out"
+    assert p.get_path("phase_path") == "./old"
+
+
+def test_groups_cover_range_and_fit():
+    # 48^3 x 96, nev 200: 6.8 GB per timeslice -> 4 groups of 24 under 200 GB
+    g = ev_ts.plan_groups(96, 48, 200, 200.0)
+    assert [n for _, n in g] == [24, 24, 24, 24]
+    assert ev_ts.batch_bytes_per_timeslice(48, 200) * 24 < 200e9
+    # 32^3 x 64, nev 220, search space 256: one group
+    assert ev_ts.plan_groups(64, 32, 220, 200.0, nsub=256) == [(0, 64)]
+    # explicit group size, ragged tail
+    assert ev_ts.plan_groups(10, 8, 20, 1.0, group=4) == [(0, 4), (4, 4), (8, 2)]
+    assert ev_ts.plan_groups(0, 8, 20, 1.0) == []
+    for todo in range(1, 40):
+        g = ev_ts.plan_groups(todo, 24, 120, 5.0)
+        assert sum(n for _, n in g) == todo and all(a == sum(n for _, n in g[:i]) for i, (a, _) in enumerate(g))

@@ -93,3 +93,28 @@ def test_batch_matches_plain_fp64(lp, gpu):
     for i in range(1, nev - 1):
         if gaps[i - 1] > 1e-4 and gaps[i] > 1e-4:
             assert np.abs(V[i] - Vpf[i]).max() < 1e-9, i
+
+
+@pytest.mark.parametrize("width", [64, 128])
+def test_filter_block_widths(lp, gpu, width, monkeypatch):
+    """Both layouts of the FP32 filter buffers (one vector per lane in
+    64-vector blocks, two per lane in 128-vector blocks) give the dense
+    oracle's spectrum, and the timing autotuner picks one of them."""
+    L, nts, nev, deg, tol = 8, 2, 60, 10, 1e-10
+    data = smeared(L, 99, 0.35, nts)
+    monkeypatch.setenv("LAPH_BW", str(width))
+    b = lp.Batch(L, deg, 12.0, 0.0, nts, nev, nsub=128, tol=tol, seed=5, device=gpu)
+    for t, (ts, _) in enumerate(data):
+        b.set_timeslice(t, ts, 0.62, 0.62, 3)
+    st = b.solve()
+    assert st.filter_block == width and st.converged == nts
+    for t, (_, U) in enumerate(data):
+        ev, V, ph, res = b.get(t)
+        check_pairs(ev, V, res, N.dense_D(U.reshape(L, L, L, 3, 3, 3)), nev, tol)
+    b.close()
+    monkeypatch.delenv("LAPH_BW")
+    b = lp.Batch(L, deg, 12.0, 0.0, nts, nev, nsub=128, tol=tol, seed=5, device=gpu)
+    for t, (ts, _) in enumerate(data):
+        b.set_timeslice(t, ts, 0.62, 0.62, 3)
+    assert b.solve().filter_block in (64, 128)
+    b.close()
