@@ -163,7 +163,10 @@ typedef struct laph_batch_opts {
   unsigned long long seed;
   int warm_start;    /* 1: start from the block proposed by laph_net_propose, when one was */
   int filter_passes; /* applications of T_DEG(B) per Rayleigh-Ritz cycle once the window has
-                        settled: 1 or 2 (0 = 2) */
+                        settled: 1 .. 6 (0 = 4) */
+  int filter_storage; /* filter vectors in HBM: 0 FP32 complex, 1 BF16 complex (half the
+                         bytes; arithmetic stays FP32, the FP64 Rayleigh-Ritz and residual
+                         test are unchanged, so only the cycle count can differ) */
 } laph_batch_opts;
 
 typedef struct laph_batch_stats {

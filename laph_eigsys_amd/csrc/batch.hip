@@ -40,6 +40,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <vector>
 
 #include "common.h"
@@ -54,14 +55,26 @@
 
 namespace laph {
 
+// Candidate block widths of the filter buffers: {64, 128} for FP32
+// storage, {128, 256} for BF16 (the narrower one divides every K).
+static void block_widths(const laph_batch* b, int cand[2]) {
+  cand[0] = b->bf16 ? 128 : 64;
+  cand[1] = b->bf16 ? 256 : 128;
+}
+
+// Filter buffer `base` at entry offset `off` (entries of esz bytes).
+static inline void* fb(const laph_batch* b, void* base, size_t off) {
+  return static_cast<char*>(base) + off * b->esz;
+}
+
 static int batch_alloc(laph_batch* b) {
   const size_t lk = (size_t)b->V3 * 27;
   const size_t nb = (size_t)b->nts * b->per_ts;
   LAPH_HIP(hipMalloc(&b->U64, lk * b->nts * sizeof(double2)));
   LAPH_HIP(hipMalloc(&b->U32, lk * b->nts * sizeof(float2)));
   for (int i = 0; i < 3; ++i) LAPH_HIP(hipMalloc(&b->B[i], nb * sizeof(double2)));
-  LAPH_HIP(hipMalloc(&b->R32, nb * sizeof(float2)));
-  for (int i = 0; i < 3; ++i) LAPH_HIP(hipMalloc(&b->P[i], nb * sizeof(float2)));
+  LAPH_HIP(hipMalloc(&b->R32, nb * b->esz));
+  for (int i = 0; i < 3; ++i) LAPH_HIP(hipMalloc(&b->P[i], nb * b->esz));
   const size_t kk = (size_t)b->nts * b->K * b->K;
   LAPH_HIP(hipMalloc(&b->G, kk * sizeof(double2)));
   LAPH_HIP(hipMalloc(&b->G2, kk * sizeof(double2)));
@@ -75,7 +88,7 @@ static int batch_alloc(laph_batch* b) {
   LAPH_HIP(hipMalloc(&b->cts, b->nts * sizeof(double)));
   LAPH_HIP(hipMalloc(&b->gam, nk * sizeof(double)));
   LAPH_HIP(hipMalloc(&b->coef, (size_t)std::max(1, b->deg - 1) * nk * sizeof(float4)));
-  LAPH_HIP(hipMalloc(&b->coef2, (size_t)b->deg * nk * sizeof(float4)));
+  LAPH_HIP(hipMalloc(&b->coef2, (size_t)(b->deg + LAPH_MAX_PASSES - 2) * nk * sizeof(float4)));
   LAPH_HIP(hipMalloc(&b->cab, b->nts * sizeof(float2)));
   LAPH_HIP(hipMalloc(&b->first_blk, b->nts * sizeof(int)));
   LAPH_HIP(hipMalloc(&b->lock_blk, b->nts * sizeof(int)));
@@ -83,18 +96,20 @@ static int batch_alloc(laph_batch* b) {
   LAPH_HIP(hipMalloc(&b->jsweeps, b->nts * sizeof(rocblas_int)));
   // defined contents for the width autotuner's first replays
   LAPH_HIP(hipMemset(b->coef, 0, (size_t)std::max(1, b->deg - 1) * nk * sizeof(float4)));
-  LAPH_HIP(hipMemset(b->coef2, 0, (size_t)b->deg * nk * sizeof(float4)));
+  LAPH_HIP(hipMemset(b->coef2, 0, (size_t)(b->deg + LAPH_MAX_PASSES - 2) * nk * sizeof(float4)));
   LAPH_HIP(hipMemset(b->cab, 0, b->nts * sizeof(float2)));
   LAPH_HIP(hipMemset(b->first_blk, 0, b->nts * sizeof(int)));
-  LAPH_HIP(hipMemset(b->R32, 0, nb * sizeof(float2)));
-  for (int i = 0; i < 3; ++i) LAPH_HIP(hipMemset(b->P[i], 0, nb * sizeof(float2)));
+  LAPH_HIP(hipMemset(b->R32, 0, nb * b->esz));
+  for (int i = 0; i < 3; ++i) LAPH_HIP(hipMemset(b->P[i], 0, nb * b->esz));
+  int cand[2];
+  block_widths(b, cand);
   if (const char* e = std::getenv("LAPH_BW")) {
     const int w = std::atoi(e);
-    LAPH_CHECK(w == 64 || (w == 128 && b->K % 128 == 0), LAPH_ERR_INVALID, "LAPH_BW=%s: 64, or 128 with K %% 128 == 0",
-               e);
+    LAPH_CHECK((w == cand[0] || w == cand[1]) && b->K % w == 0, LAPH_ERR_INVALID,
+               "LAPH_BW=%s: %d or %d dividing K=%d", e, cand[0], cand[1], b->K);
     b->bw = w;
-  } else if (b->K % 128 != 0) {
-    b->bw = 64;
+  } else if (b->K % cand[1] != 0) {
+    b->bw = cand[0];
   }
   int nlanes = 2;
   if (const char* e = std::getenv("LAPH_LANES")) nlanes = std::max(1, std::min(4, std::atoi(e)));
@@ -120,7 +135,8 @@ static void batch_free(laph_batch* b) {
   for (auto& ln : b->lanes) {
     if (ln.st) (void)hipStreamSynchronize(ln.st);
     if (ln.graph) (void)hipGraphExecDestroy(ln.graph);
-    if (ln.graph2) (void)hipGraphExecDestroy(ln.graph2);
+    for (auto g : ln.gplain)
+      if (g) (void)hipGraphExecDestroy(g);
     for (auto e : ln.ev)
       if (e) (void)hipEventDestroy(e);
     if (ln.rb) rocblas_destroy_handle(ln.rb);
@@ -327,8 +343,8 @@ static int residuals_and_window(const View& v, std::vector<double>& lc, int* n_a
   }
   LAPH_HIP(hipMemcpyAsync(b->lam + v.nk(), hl.data(), hl.size() * sizeof(double), hipMemcpyHostToDevice, ln->st));
   LAPH_HIP(hipMemcpyAsync(b->cts + ln->t0, hc.data(), hc.size() * sizeof(double), hipMemcpyHostToDevice, ln->st));
-  LAPH_HIP(launch_residual(v.X(), v.AX(), b->lam + v.nk(), b->cts + ln->t0, b->R32 + v.vo(), b->nrm2 + v.nk(), b->n,
-                           K, nt, ln->lock, b->bw, ln->st));
+  LAPH_HIP(launch_residual(v.X(), v.AX(), b->lam + v.nk(), b->cts + ln->t0, fb(b, b->R32, v.vo()), b->nrm2 + v.nk(),
+                           b->n, K, nt, ln->lock, b->bw, b->bf16, ln->st));
   LAPH_HIP(hipMemcpyAsync(b->h_res.data() + v.nk(), b->nrm2 + v.nk(), (size_t)nt * K * sizeof(double),
                           hipMemcpyDeviceToHost, ln->st));
   LAPH_HIP(hipStreamSynchronize(ln->st));
@@ -355,13 +371,15 @@ static int residuals_and_window(const View& v, std::vector<double>& lc, int* n_a
   return LAPH_OK;
 }
 
-// Per-column coefficients of the m-1 filter steps and the combine factor.
-static int upload_coefficients(const View& v, const std::vector<double>& lc) {
+// Per-column coefficients of the filter steps of a cycle of `passes`
+// applications of T_m(B), and the combine factor gamma^passes.
+static int upload_coefficients(const View& v, const std::vector<double>& lc, int passes) {
   laph_batch* b = v.b;
   laph_lane* ln = v.ln;
   const int K = b->K, nt = ln->nt, m = b->deg;
+  const int n2 = m + LAPH_MAX_PASSES - 2;  // plain-pass tables: m-1 shared steps + one last step per pass
   const double lamL = b->p.lambda_l;
-  std::vector<float4> co((size_t)(m - 1) * nt * K), co2((size_t)m * nt * K);
+  std::vector<float4> co((size_t)(m - 1) * nt * K), co2((size_t)n2 * nt * K);
   std::vector<double> ga((size_t)nt * K);
   std::vector<float2> cab(nt);
   std::vector<int> fb(nt);
@@ -391,20 +409,26 @@ static int upload_coefficients(const View& v, const std::vector<double>& lc) {
         }
         co[((size_t)(st - 1) * nt + t) * K + j] = q;
       }
-      // plain pass h_k = T_k(B) f / s_k: h_1 = B f / s_1,
-      // h_{k+1} = (2 s_k/s_{k+1}) B h_k - (s_{k-1}/s_{k+1}) h_{k-1}, h_0 = f
+      // plain pass p on w = w_{p-1}: h_k = T_k(B) w / s_k with h_0 = w,
+      // h_1 = B w / s_1, h_{k+1} = (2 s_k/s_{k+1}) B h_k - (s_{k-1}/s_{k+1}) h_{k-1};
+      // its last step also adds gamma^(p-1) f_m (source slot), giving w_p
+      const double g = tau[m] / s[m];
       co2[((size_t)0 * nt + t) * K + j] = make_float4((float)(1.0 / s[1]), 0.f, 0.f, 0.f);
-      for (int st = 2; st <= m; ++st)
+      for (int st = 2; st < m; ++st)
         co2[((size_t)(st - 1) * nt + t) * K + j] =
             make_float4((float)(2.0 * s[st - 1] / s[st]), 0.f, (float)(-s[st - 2] / s[st]), 0.f);
-      ga[(size_t)t * K + j] = tau[m] / s[m];
+      double gp = g;
+      for (int pp = 2; pp <= LAPH_MAX_PASSES; ++pp, gp *= g)
+        co2[((size_t)(m - 1 + pp - 2) * nt + t) * K + j] =
+            make_float4((float)(2.0 * s[m - 1] / s[m]), (float)gp, (float)(-s[m - 2] / s[m]), 0.f);
+      ga[(size_t)t * K + j] = std::pow(g, passes);
     }
   }
   // coefficient table of the lane: [step][lane timeslice][K] at offset step*nts*K + t0*K
   for (int st = 1; st < m; ++st)
     LAPH_HIP(hipMemcpyAsync(b->coef + (size_t)(st - 1) * b->nts * K + v.nk(), co.data() + (size_t)(st - 1) * nt * K,
                             (size_t)nt * K * sizeof(float4), hipMemcpyHostToDevice, ln->st));
-  for (int st = 1; st <= m; ++st)
+  for (int st = 1; st <= n2; ++st)
     LAPH_HIP(hipMemcpyAsync(b->coef2 + (size_t)(st - 1) * b->nts * K + v.nk(), co2.data() + (size_t)(st - 1) * nt * K,
                             (size_t)nt * K * sizeof(float4), hipMemcpyHostToDevice, ln->st));
   LAPH_HIP(hipMemcpyAsync(b->gam + v.nk(), ga.data(), ga.size() * sizeof(double), hipMemcpyHostToDevice, ln->st));
@@ -419,52 +443,74 @@ static int enqueue_filter(const View& v) {
   const int m = b->deg, K = b->K, nt = v.ln->nt, L = b->p.L;
   const size_t step = (size_t)b->nts * K;
   const size_t vo = v.vo();
-  float2* R = b->R32 + vo;
+  void* R = fb(b, b->R32, vo);
   for (int st = 1; st < m; ++st) {
-    const float2* in = st == 1 ? R : b->P[(st - 2) % 3] + vo;
-    const float2* old = st >= 3 ? b->P[(st - 3) % 3] + vo : nullptr;
-    float2* out = b->P[(st - 1) % 3] + vo;
-    LAPH_HIP(launch_filter_step32(b->U32 + v.lo(), in, R, old, out, b->coef + (st - 1) * step + v.nk(),
-                                  b->cab + v.ln->t0, b->first_blk + v.ln->t0, L, K, nt, b->bw, v.ln->st));
+    const void* in = st == 1 ? R : fb(b, b->P[(st - 2) % 3], vo);
+    const void* old = st >= 3 ? fb(b, b->P[(st - 3) % 3], vo) : nullptr;
+    void* out = fb(b, b->P[(st - 1) % 3], vo);
+    LAPH_HIP(launch_filter_step(b->U32 + v.lo(), in, R, old, out, b->coef + (st - 1) * step + v.nk(),
+                                b->cab + v.ln->t0, b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, v.ln->st));
   }
   return LAPH_OK;
 }
 
-// Buffers of the plain pass: f_m stays where the correction pass left it
-// (Pf); h_k rotates over the other three buffers (R32 is free by then).
-static void plain_buffers(const View& v, float2** pf, float2* q[3]) {
+// Buffers of the plain passes: f = f_m stays where the correction pass left
+// it (every plain pass adds gamma^(p-1) f in its last step); the recurrences
+// rotate over the other three buffers q[] (R32 is free by then).
+static void plain_buffers(const View& v, void** pf, void* q[3]) {
   laph_batch* b = v.b;
   const int m = b->deg;
-  float2* all[4] = {b->R32, b->P[0], b->P[1], b->P[2]};
-  float2* f = b->P[(m - 2) % 3];
+  void* all[4] = {b->R32, b->P[0], b->P[1], b->P[2]};
+  void* f = b->P[(m - 2) % 3];
   int k = 0;
-  for (float2* a : all)
-    if (a != f) q[k++] = a + v.vo();
-  *pf = f + v.vo();
+  for (void* a : all)
+    if (a != f) q[k++] = fb(b, a, v.vo());
+  *pf = fb(b, f, v.vo());
 }
 
-// The m plain steps h_1 .. h_m applied to f_m.
-static int enqueue_filter_plain(const View& v) {
+// Slot of q[] holding w_p after plain pass p >= 2.
+static int plain_result(int m, int p) {
+  int r = (m - 1) % 3;
+  for (int k = 3; k <= p; ++k) r = (r + m) % 3;
+  return r;
+}
+
+// Plain pass p >= 2: w_p = T_m(B) w_{p-1} / s_m + gamma^(p-1) f (w_1 = f).
+// Pass 2 reads f as h_0; pass p > 2 starts from q[a], a = where pass p-1
+// ended, and rotates q[a], q[a+1], q[a+2] so h_0 is overwritten only after
+// its last use (step 2).
+static int enqueue_plain_pass(const View& v, int p) {
   laph_batch* b = v.b;
   const int m = b->deg, K = b->K, nt = v.ln->nt, L = b->p.L;
   const size_t step = (size_t)b->nts * K;
-  float2 *f, *q[3];
+  void *f, *q[3];
   plain_buffers(v, &f, q);
+  const int a = p == 2 ? 0 : plain_result(m, p - 1);
   for (int st = 1; st <= m; ++st) {
-    const float2* in = st == 1 ? f : q[(st - 2) % 3];
-    const float2* old = st == 1 ? nullptr : (st == 2 ? f : q[(st - 3) % 3]);
-    float2* out = q[(st - 1) % 3];
-    LAPH_HIP(launch_filter_step32(b->U32 + v.lo(), in, nullptr, old, out, b->coef2 + (st - 1) * step + v.nk(),
-                                  b->cab + v.ln->t0, b->first_blk + v.ln->t0, L, K, nt, b->bw, v.ln->st));
+    const void *in, *old;
+    void* out;
+    if (p == 2) {
+      in = st == 1 ? f : q[(st - 2) % 3];
+      old = st == 1 ? nullptr : (st == 2 ? f : q[(st - 3) % 3]);
+      out = q[(st - 1) % 3];
+    } else {
+      in = q[(a + st - 1) % 3];
+      old = st >= 2 ? q[(a + st - 2) % 3] : nullptr;
+      out = q[(a + st) % 3];
+    }
+    const bool last = st == m;
+    const float4* co = b->coef2 + (size_t)(last ? m - 1 + p - 2 : st - 1) * step + v.nk();
+    LAPH_HIP(launch_filter_step(b->U32 + v.lo(), in, last ? f : nullptr, old, out, co, b->cab + v.ln->t0,
+                                b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, v.ln->st));
   }
   return LAPH_OK;
 }
 
-static int capture(const View& v, hipGraphExec_t* exec, int (*enqueue)(const View&)) {
+static int capture(const View& v, hipGraphExec_t* exec, const std::function<int()>& enqueue) {
   laph_lane* ln = v.ln;
   hipGraph_t g;
   LAPH_HIP(hipStreamBeginCapture(ln->st, hipStreamCaptureModeThreadLocal));
-  int rc = enqueue(v);
+  int rc = enqueue();
   hipError_t e = hipStreamEndCapture(ln->st, &g);
   LAPH_TRY(rc);
   LAPH_HIP(e);
@@ -473,29 +519,42 @@ static int capture(const View& v, hipGraphExec_t* exec, int (*enqueue)(const Vie
   return LAPH_OK;
 }
 
-// Replays the captured correction pass of the lane (captured on first use);
-// with passes == 2 also the plain pass.  *fm <- f_m, *hm <- h_m (or null).
-static int run_filter(const View& v, int passes, float2** fm, float2** hm) {
+// Replays the captured correction pass of the lane and passes 2..`passes`
+// (each captured on first use).  *w <- w_passes (f_m for one pass), with
+// T_m(B)^passes x / s_m^passes = gamma^passes x + w_passes.
+static int run_filter(const View& v, int passes, void** w) {
   laph_lane* ln = v.ln;
-  if (!ln->graph) LAPH_TRY(capture(v, &ln->graph, enqueue_filter));
-  *fm = v.b->P[(v.b->deg - 2) % 3] + v.vo();
-  *hm = nullptr;
+  if (!ln->graph) LAPH_TRY(capture(v, &ln->graph, [&] { return enqueue_filter(v); }));
+  for (int p = 2; p <= passes; ++p)
+    if (!ln->gplain[p - 2]) LAPH_TRY(capture(v, &ln->gplain[p - 2], [&] { return enqueue_plain_pass(v, p); }));
   LAPH_HIP(hipGraphLaunch(ln->graph, ln->st));
-  if (passes >= 2) {
-    if (!ln->graph2) LAPH_TRY(capture(v, &ln->graph2, enqueue_filter_plain));
-    float2 *f, *q[3];
-    plain_buffers(v, &f, q);
-    *hm = q[(v.b->deg - 1) % 3];
-    LAPH_HIP(hipGraphLaunch(ln->graph2, ln->st));
+  void *f, *q[3];
+  plain_buffers(v, &f, q);
+  *w = f;
+  for (int p = 2; p <= passes; ++p) {
+    LAPH_HIP(hipGraphLaunch(ln->gplain[p - 2], ln->st));
+    *w = q[plain_result(v.b->deg, p)];
   }
   return LAPH_OK;
 }
 
-// One pass:  y = gamma x + f_m                 (= T(B) x / s_m)
-// Two passes: y = gamma^2 x + gamma f_m + h_m    (= T(B)^2 x / s_m^2)
-__global__ void k_combine_gamma(double2* __restrict__ X, const float2* __restrict__ F, const float2* __restrict__ Hm,
-                                const double* __restrict__ gam, size_t per_ts, int K,
-                                const int* __restrict__ first_blk, int bw) {
+template <class E>
+__device__ __forceinline__ float2 load_entry(const E* p, size_t i);
+template <>
+__device__ __forceinline__ float2 load_entry<float2>(const float2* p, size_t i) {
+  return p[i];
+}
+template <>
+__device__ __forceinline__ float2 load_entry<unsigned>(const unsigned* p, size_t i) {
+  const unsigned u = p[i];
+  return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u));
+}
+
+// X <- g X + w  (g = gamma^passes per column, w block-major in the filter
+// storage: E = float2 for FP32, unsigned for a BF16 complex pair)
+template <class E>
+__global__ void k_combine_gamma(double2* __restrict__ X, const E* __restrict__ W, const double* __restrict__ gam,
+                                size_t per_ts, int K, const int* __restrict__ first_blk, int bw) {
   const int ts = blockIdx.y;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= per_ts) return;
@@ -503,16 +562,17 @@ __global__ void k_combine_gamma(double2* __restrict__ X, const float2* __restric
   if (col < first_blk[ts] * 64) return;
   const size_t idx = (size_t)ts * per_ts + i;
   const double g = gam[(size_t)ts * K + col];
-  const size_t row = i / K;  // F is block-major: [ts][col/bw][row][bw]
-  const size_t fi = (size_t)ts * per_ts + ((size_t)(col / bw) * (per_ts / K) + row) * bw + (col % bw);
-  const float2 f = F[fi];
+  const size_t row = i / K;  // W is block-major: [ts][col/bw][row][bw]
+  const size_t wi = (size_t)ts * per_ts + ((size_t)(col / bw) * (per_ts / K) + row) * bw + (col % bw);
+  const float2 w = load_entry(W, wi);
   const double2 x = X[idx];
-  if (Hm) {
-    const float2 h = Hm[fi];
-    X[idx] = make_double2(g * (g * x.x + (double)f.x) + (double)h.x, g * (g * x.y + (double)f.y) + (double)h.y);
-  } else {
-    X[idx] = make_double2(g * x.x + (double)f.x, g * x.y + (double)f.y);
-  }
+  X[idx] = make_double2(g * x.x + (double)w.x, g * x.y + (double)w.y);
+}
+
+// Applications of T_m(B) per cycle once the window has settled.
+static int filter_passes(const laph_batch* b) {
+  const int p = b->o.filter_passes > 0 ? b->o.filter_passes : 4;
+  return std::min(p, LAPH_MAX_PASSES);
 }
 
 // Solve loop of one lane (runs on its own host thread).
@@ -540,36 +600,44 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
   const int max_iter = b->o.max_iter > 0 ? b->o.max_iter : 60;
   int it = 0;
   for (it = 0; it < max_iter && active > 0; ++it) {
-    // one filter pass while the window settles, then T_m(B)^2 per cycle
-    const int passes = it < 2 ? 1 : std::max(1, b->o.filter_passes > 0 ? std::min(b->o.filter_passes, 2) : 2);
-    LAPH_TRY(upload_coefficients(v, lc));
+    // one filter pass while the window settles, then T_m(B)^P per cycle
+    const int passes = it < b->warm_cycles ? 1 : filter_passes(b);
+    LAPH_TRY(upload_coefficients(v, lc, passes));
     for (int t = 0; t < nt; ++t) {
       const int nl = b->nlock[ln->t0 + t];
       if (nl >= b->nev) continue;
       const int cols = K - (nl / 64) * 64;
       const int nblk = (cols + b->bw - 1) / b->bw;  // blocks the kernel visits (links once each)
       ln->sv32 += (long long)(b->deg - 1) * cols;
-      // algorithmic bytes: step 1 reads r and writes f_2 (2 vector touches),
-      // step 2 adds r (3), later steps add f_{k-1} (4); a vector touch is
-      // 3 colours x 8 B per site; links 27 x 8 B per site per 64-vector block
+      // algorithmic bytes: correction step 1 reads r and writes f_2 (2 vector
+      // touches), step 2 adds r (3), later steps add f_{k-1} (4); plain-pass
+      // step 1 in/out (2), later steps in/old/out (3), the last one also f (4).
+      // A vector touch is 3 colours x esz bytes per site; links 27 x 8 B per
+      // site per vector block
       for (int s = 1; s < b->deg; ++s) {
         const int touches = s == 1 ? 2 : (s == 2 ? 3 : 4);
-        ln->fbytes += (double)b->V3 * ((double)cols * touches * 24.0 + nblk * 216.0);
+        ln->fbytes += (double)b->V3 * ((double)cols * touches * 3.0 * b->esz + nblk * 216.0);
       }
-      if (passes == 2) {  // plain pass: step 1 in/out, later steps in/old/out
+      for (int p = 2; p <= passes; ++p) {
         ln->sv32 += (long long)b->deg * cols;
-        for (int s = 1; s <= b->deg; ++s)
-          ln->fbytes += (double)b->V3 * ((double)cols * (s == 1 ? 2 : 3) * 24.0 + nblk * 216.0);
+        for (int s = 1; s <= b->deg; ++s) {
+          const int touches = s == 1 ? 2 : (s == b->deg ? 4 : 3);
+          ln->fbytes += (double)b->V3 * ((double)cols * touches * 3.0 * b->esz + nblk * 216.0);
+        }
       }
     }
-    ln->flaunch += (b->deg - 1) + (passes == 2 ? b->deg : 0);
-    float2 *fm = nullptr, *hm = nullptr;
+    ln->flaunch += (b->deg - 1) + (long long)(passes - 1) * b->deg;
+    void* w = nullptr;
     LAPH_HIP(hipEventRecord(ln->ev[0], ln->st));
-    LAPH_TRY(run_filter(v, passes, &fm, &hm));
+    LAPH_TRY(run_filter(v, passes, &w));
     LAPH_HIP(hipEventRecord(ln->ev[1], ln->st));
     dim3 g((unsigned)((b->per_ts + 255) / 256), nt);
-    k_combine_gamma<<<g, 256, 0, ln->st>>>(v.X(), fm, hm, b->gam + v.nk(), b->per_ts, K, b->first_blk + ln->t0,
-                                           b->bw);
+    if (b->bf16)
+      k_combine_gamma<unsigned><<<g, 256, 0, ln->st>>>(v.X(), static_cast<const unsigned*>(w), b->gam + v.nk(),
+                                                      b->per_ts, K, b->first_blk + ln->t0, b->bw);
+    else
+      k_combine_gamma<float2><<<g, 256, 0, ln->st>>>(v.X(), static_cast<const float2*>(w), b->gam + v.nk(),
+                                                    b->per_ts, K, b->first_blk + ln->t0, b->bw);
     LAPH_HIP(hipGetLastError());
     const Act q{ln->lock, K - ln->lock};
     LAPH_TRY(project_locked(v, q));
@@ -608,7 +676,7 @@ static int time_correction_pass(laph_batch* b, int reps, float* ms) {
   const View v{b, ln};
   if (ln->graph) LAPH_HIP(hipGraphExecDestroy(ln->graph));
   ln->graph = nullptr;
-  LAPH_TRY(capture(v, &ln->graph, enqueue_filter));
+  LAPH_TRY(capture(v, &ln->graph, [&] { return enqueue_filter(v); }));
   LAPH_HIP(hipGraphLaunch(ln->graph, ln->st));  // warm-up
   LAPH_HIP(hipEventRecord(ln->ev[0], ln->st));
   for (int r = 0; r < reps; ++r) LAPH_HIP(hipGraphLaunch(ln->graph, ln->st));
@@ -624,13 +692,16 @@ static int time_correction_pass(laph_batch* b, int reps, float* ms) {
 static int autotune_block_width(laph_batch* b) {
   laph_lane* ln = &b->lanes[0];
   LAPH_HIP(hipMemsetAsync(b->first_blk + ln->t0, 0, ln->nt * sizeof(int), ln->st));
+  int cand[2];
+  block_widths(b, cand);
   float best = 0;
-  int wbest = 64;
-  for (int w : {64, 128}) {
+  int wbest = cand[0];
+  for (int w : cand) {
+    if (b->K % w != 0) continue;
     b->bw = w;
     float ms = 0;
     LAPH_TRY(time_correction_pass(b, 3, &ms));
-    if (w == 64 || ms < best) {
+    if (w == cand[0] || ms < best) {
       best = ms;
       wbest = w;
     }
@@ -705,6 +776,10 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
   LAPH_CHECK(p->lambda_l > 0, LAPH_ERR_INVALID, "laph_batch_create: lambda_l must be positive");
   LAPH_CHECK(o->nts >= 1 && o->nev >= 1, LAPH_ERR_INVALID, "laph_batch_create: nts=%d nev=%d", o->nts, o->nev);
   LAPH_CHECK(o->tol > 0, LAPH_ERR_INVALID, "laph_batch_create: tol must be positive");
+  LAPH_CHECK(o->filter_passes >= 0 && o->filter_passes <= LAPH_MAX_PASSES, LAPH_ERR_INVALID,
+             "laph_batch_create: filter_passes=%d outside 0..%d", o->filter_passes, LAPH_MAX_PASSES);
+  LAPH_CHECK(o->filter_storage == 0 || o->filter_storage == 1, LAPH_ERR_INVALID,
+             "laph_batch_create: filter_storage=%d (0 FP32, 1 BF16)", o->filter_storage);
   const int n = 3 * p->L * p->L * p->L;
   int K = o->nsub > 0 ? o->nsub : o->nev + std::max(16, o->nev / 4);
   K = (K + 63) / 64 * 64;
@@ -725,6 +800,9 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     b->deg = p->deg;
     b->per_ts = (size_t)n * K;
     b->trace = std::getenv("LAPH_TRACE") != nullptr;
+    if (const char* e = std::getenv("LAPH_WARM")) b->warm_cycles = std::max(0, std::atoi(e));
+    b->bf16 = o->filter_storage == 1;
+    b->esz = b->bf16 ? 4 : 8;
     if (const char* e = std::getenv("LAPH_EIG")) b->eig_mode = std::atoi(e);
     b->nlock.assign(b->nts, 0);
     b->h_mu.assign((size_t)b->nts * K, 0.0);
@@ -802,10 +880,10 @@ int laph_batch_time_filter(laph_batch* b, int reps, double* ms_per_launch, doubl
     // every column of every timeslice of lane 0 active
     std::vector<int> zero(nt, 0);
     LAPH_HIP(hipMemcpyAsync(b->first_blk + ln->t0, zero.data(), nt * sizeof(int), hipMemcpyHostToDevice, ln->st));
-    float2 *fm = nullptr, *hm = nullptr;
-    LAPH_TRY(laph::run_filter(v, 1, &fm, &hm));  // warm-up (and capture if needed)
+    void* w = nullptr;
+    LAPH_TRY(laph::run_filter(v, 1, &w));  // warm-up (and capture if needed)
     LAPH_HIP(hipEventRecord(ln->ev[0], ln->st));
-    for (int r = 0; r < reps; ++r) LAPH_TRY(laph::run_filter(v, 1, &fm, &hm));
+    for (int r = 0; r < reps; ++r) LAPH_TRY(laph::run_filter(v, 1, &w));
     LAPH_HIP(hipEventRecord(ln->ev[1], ln->st));
     LAPH_HIP(hipEventSynchronize(ln->ev[1]));
     float ms = 0;
@@ -814,7 +892,7 @@ int laph_batch_time_filter(laph_batch* b, int reps, double* ms_per_launch, doubl
     double bytes = 0;
     for (int s = 1; s < b->deg; ++s) {
       const int touches = s == 1 ? 2 : (s == 2 ? 3 : 4);
-      bytes += (double)nt * b->V3 * ((double)K * touches * 24.0 + (K / b->bw) * 216.0);
+      bytes += (double)nt * b->V3 * ((double)K * touches * 3.0 * b->esz + (K / b->bw) * 216.0);
     }
     *ms_per_launch = ms / ((double)reps * steps);
     *bytes_per_launch = bytes / steps;

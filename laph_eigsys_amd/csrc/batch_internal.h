@@ -20,14 +20,17 @@
 // host thread on its own HIP stream and rocBLAS handle.  Two lanes run side
 // by side so the HBM-bound FP32 filter of one overlaps the MFMA-bound dense
 // FP64 steps of the other.
+// Applications of T_DEG(B) per Rayleigh-Ritz cycle at most.
+#define LAPH_MAX_PASSES 6
+
 struct laph_lane {
   int t0 = 0, nt = 0;
   hipStream_t st = nullptr;
   rocblas_handle rb = nullptr;
   int iX = 0, iAX = 1, iT = 2;  // roles of the three FP64 block buffers
   int lock = 0;                 // hard-locked columns, common to the lane
-  hipGraphExec_t graph = nullptr;   // correction-form filter pass
-  hipGraphExec_t graph2 = nullptr;  // plain filter pass (second application)
+  hipGraphExec_t graph = nullptr;  // correction-form filter pass
+  hipGraphExec_t gplain[LAPH_MAX_PASSES - 1] = {};  // plain passes 2 .. LAPH_MAX_PASSES
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   // per-solve statistics
   int cycles = 0, rc = 0;
@@ -50,8 +53,10 @@ struct laph_batch {
   double2* Unew = nullptr;
   // blocks [nts][n][K]; after a solve the eigenvectors are in B[0]
   double2* B[3] = {nullptr, nullptr, nullptr};
-  float2* R32 = nullptr;
-  float2* P[3] = {nullptr, nullptr, nullptr};
+  // filter buffers: esz bytes per complex entry (8: FP32, 4: BF16 storage)
+  void* R32 = nullptr;
+  void* P[3] = {nullptr, nullptr, nullptr};
+  int esz = 8;
   // small per-timeslice data
   double2* G = nullptr;      // [nts][K*K] Rayleigh-Ritz / CholQR matrix
   double2* G2 = nullptr;     // [nts][K*K] Gram of the active block
@@ -65,7 +70,7 @@ struct laph_batch {
   double* cts = nullptr;     // [nts]
   double* gam = nullptr;     // [nts][K]
   float4* coef = nullptr;    // [deg-1][nts][K]
-  float4* coef2 = nullptr;   // [deg][nts][K] plain pass
+  float4* coef2 = nullptr;   // [deg - 1 + LAPH_MAX_PASSES - 1][nts][K] plain passes
   float2* cab = nullptr;     // [nts]
   int* first_blk = nullptr;  // [nts] first filtered 64-column block
   int* lock_blk = nullptr;   // [nts] first active 64-column block of the dense steps
@@ -81,5 +86,7 @@ struct laph_batch {
   // vectors per block of the FP32 filter buffers (64 or 128); 0 until the
   // first solve times both widths (LAPH_BW=64|128 fixes it)
   int bw = 0;
+  int bf16 = 0;  // filter vectors stored as BF16 (laph_batch_opts.filter_storage)
+  int warm_cycles = 2;  // single-pass cycles while the window settles (LAPH_WARM)
 };
 

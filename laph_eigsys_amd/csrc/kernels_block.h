@@ -10,18 +10,20 @@ namespace laph {
 // coef[ts*K + col] = (a1, a2, b, -), cab[ts] = (c_t, a_t - 6 c_t); column
 // blocks below first_blk[ts] are skipped.  old and r may be null (plain
 // recurrence without the source term).
-// bw = 64: one vector per lane; bw = 128: two vectors per lane (float4),
-// buffers block-major with bw-vector blocks.
-hipError_t launch_filter_step32(const float2* U32, const float2* in, const float2* r, const float2* old,
-                                float2* out, const float4* coef, const float2* cab, const int* first_blk,
-                                int L, int K, int nts, int bw, hipStream_t st);
+// Filter vectors stored as FP32 complex (bf16 = 0; bw = 64: one vector per
+// lane, 128: two per lane) or as BF16 complex (bf16 = 1; bw = 128 or 256:
+// two or four per lane); buffers block-major with bw-vector blocks.
+hipError_t launch_filter_step(const float2* U32, const void* in, const void* r, const void* old, void* out,
+                              const float4* coef, const float2* cab, const int* first_blk, int L, int K, int nts,
+                              int bw, int bf16, hipStream_t st);
 // out = D v on every (or, with first_blk, every active) column.
 hipError_t launch_lap_block64(const double2* U64, const double2* v, double2* out, int L, int K, int nts,
                               const int* first_blk, hipStream_t st);
 // R = AX - X diag(lam), r32 = c_t R, nrm2 = column sums of |R|^2, for the
 // columns >= first_col (a multiple of 64).
 hipError_t launch_residual(const double2* X, const double2* AX, const double* lam, const double* cts,
-                           float2* r32, double* nrm2, int n, int K, int nts, int first_col, int bw, hipStream_t st);
+                           void* r, double* nrm2, int n, int K, int nts, int first_col, int bw, int bf16,
+                           hipStream_t st);
 // Column-major, phase-fixed copy of the first nev columns of one timeslice.
 hipError_t launch_extract_phase(const double2* X, double2* V, double* phase, int n, int K, int nev,
                                 hipStream_t st);

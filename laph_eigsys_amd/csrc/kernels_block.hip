@@ -202,6 +202,7 @@ __global__ __launch_bounds__(256) void k_filter_step32x2(const float2* __restric
   if (s >= V3) return;
   const int l64 = threadIdx.x & 63;
   const int col = cb * 128 + 2 * l64;
+  if (col < first_blk[ts] * 64) return;  // lanes of a locked 64-vector half issue no traffic
   const size_t vo = ((size_t)ts * (K / 128) + cb) * (size_t)V3 * 3 * 64;  // in float4 units
   const float2* Ut = U32 + (size_t)ts * V3 * 27;
   const int L2 = L * L;
@@ -269,6 +270,142 @@ __global__ __launch_bounds__(256) void k_filter_step32x2(const float2* __restric
   }
 }
 
+// BF16 storage of the filter vectors: a complex value is two bfloat16 in one
+// 32-bit word (real part in the low half); all arithmetic stays FP32.
+__device__ __forceinline__ float2 bf2_unpack(unsigned u) {
+  return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u));
+}
+__device__ __forceinline__ unsigned bf_round(float f) {  // to nearest, ties to even
+  const unsigned a = __float_as_uint(f);
+  return (a + 0x7fffu + ((a >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ unsigned bf2_pack(float2 v) { return bf_round(v.x) | (bf_round(v.y) << 16); }
+
+template <int VPL>
+struct UVec;
+template <>
+struct UVec<2> {
+  typedef unsigned T __attribute__((ext_vector_type(2)));
+};
+template <>
+struct UVec<4> {
+  typedef unsigned T __attribute__((ext_vector_type(4)));
+};
+
+// The filter step on BF16-stored vectors: each lane carries VPL neighbouring
+// vectors of a BW = 64*VPL block in one 8- or 16-byte word.
+template <bool HAS_R, int VPL>
+__global__ __launch_bounds__(256) void k_filter_step_bf16(const float2* __restrict__ U32,
+                                                           const typename UVec<VPL>::T* __restrict__ in,
+                                                           const typename UVec<VPL>::T* __restrict__ r,
+                                                           const typename UVec<VPL>::T* __restrict__ old,
+                                                           typename UVec<VPL>::T* __restrict__ out,
+                                                           const float4* __restrict__ coef,
+                                                           const float2* __restrict__ cab,
+                                                           const int* __restrict__ first_blk, int L, int K,
+                                                           int nts, int use_old) {
+  typedef typename UVec<VPL>::T T;
+  constexpr int BW = 64 * VPL;
+  const int V3 = L * L * L;
+  int ts, cb, sb;
+  unit_map((V3 + 3) / 4, K / BW, nts, ts, cb, sb);
+  if ((cb + 1) * VPL <= first_blk[ts]) return;  // every 64-vector part of the block locked
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int s = sb * 4 + wave;
+  if (s >= V3) return;
+  const int l64 = threadIdx.x & 63;
+  const int col = cb * BW + VPL * l64;
+  if (col < first_blk[ts] * 64) return;  // lanes of locked 64-vector parts issue no traffic
+  const size_t vo = ((size_t)ts * (K / BW) + cb) * (size_t)V3 * 3 * 64;  // in words of T
+  const float2* Ut = U32 + (size_t)ts * V3 * 27;
+  const int L2 = L * L;
+  const int x = s / L2, y = (s / L) % L, z = s - (s / L) * L;
+  const int up[3] = {x == L - 1 ? s - (L - 1) * L2 : s + L2, y == L - 1 ? s - (L - 1) * L : s + L,
+                     z == L - 1 ? s - (L - 1) : s + 1};
+  const int dn[3] = {x == 0 ? s + (L - 1) * L2 : s - L2, y == 0 ? s + (L - 1) * L : s - L,
+                     z == 0 ? s + (L - 1) : s - 1};
+  const T* v = in + vo;
+  float2 acc[3][VPL];
+  T ctr[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    ctr[c] = v[(size_t)(3 * s + c) * 64 + l64];
+#pragma unroll
+    for (int j = 0; j < VPL; ++j) acc[c][j] = make_float2(0.f, 0.f);
+  }
+#pragma unroll
+  for (int mu = 0; mu < 3; ++mu) {
+    T xu[3], xm[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      xu[c] = v[(size_t)(3 * up[mu] + c) * 64 + l64];
+      xm[c] = v[(size_t)(3 * dn[mu] + c) * 64 + l64];
+    }
+    const float2* Uf = Ut + ((size_t)s * 3 + mu) * 9;
+    const float2* Ub = Ut + ((size_t)dn[mu] * 3 + mu) * 9;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+      for (int bb = 0; bb < 3; ++bb) {
+        const float2 uf = Uf[3 * a + bb], ub = Ub[3 * bb + a];
+#pragma unroll
+        for (int j = 0; j < VPL; ++j) {
+          acc[a][j] = cfma(uf, bf2_unpack(xu[bb][j]), acc[a][j]);
+          acc[a][j] = cfmac(ub, bf2_unpack(xm[bb][j]), acc[a][j]);
+        }
+      }
+    }
+  }
+  float4 cf[VPL];
+#pragma unroll
+  for (int j = 0; j < VPL; ++j) cf[j] = coef[(size_t)ts * K + col + j];
+  const float2 ca = cab[ts];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t idx = vo + (size_t)(3 * s + c) * 64 + l64;
+    T rv, ov, o;
+    if (HAS_R) rv = __builtin_nontemporal_load(r + idx);
+    if (use_old) ov = __builtin_nontemporal_load(old + idx);
+#pragma unroll
+    for (int j = 0; j < VPL; ++j) {
+      const float2 xc = bf2_unpack(ctr[c][j]);
+      float2 val = make_float2(cf[j].x * (ca.x * acc[c][j].x + ca.y * xc.x),
+                               cf[j].x * (ca.x * acc[c][j].y + ca.y * xc.y));
+      if (HAS_R) {
+        const float2 rr = bf2_unpack(rv[j]);
+        val.x += cf[j].y * rr.x;
+        val.y += cf[j].y * rr.y;
+      }
+      if (use_old) {
+        const float2 oo = bf2_unpack(ov[j]);
+        val.x += cf[j].z * oo.x;
+        val.y += cf[j].z * oo.y;
+      }
+      o[j] = bf2_pack(val);
+    }
+    __builtin_nontemporal_store(o, out + idx);
+  }
+}
+
+template <int VPL>
+static void launch_bf16(const float2* U32, const void* in, const void* r, const void* old, void* out,
+                        const float4* coef, const float2* cab, const int* first_blk, int L, int K, int nts,
+                        hipStream_t st) {
+  typedef typename UVec<VPL>::T T;
+  const int V3 = L * L * L;
+  dim3 g((unsigned)((V3 + 3) / 4) * (unsigned)(K / (64 * VPL)) * (unsigned)nts);
+  const T* i4 = static_cast<const T*>(in);
+  const T* r4 = static_cast<const T*>(r);
+  const T* o4 = static_cast<const T*>(old);
+  T* out4 = static_cast<T*>(out);
+  if (r)
+    k_filter_step_bf16<true, VPL><<<g, 256, 0, st>>>(U32, i4, r4, o4, out4, coef, cab, first_blk, L, K, nts,
+                                                     old != nullptr);
+  else
+    k_filter_step_bf16<false, VPL><<<g, 256, 0, st>>>(U32, i4, r4, o4, out4, coef, cab, first_blk, L, K, nts,
+                                                      old != nullptr);
+}
+
 // FP64 Laplacian on a block: out = H v - 6 v (= D v) for all columns.
 __global__ __launch_bounds__(256) void k_lap_block64(const double2* __restrict__ U64,
                                                       const double2* __restrict__ v,
@@ -301,9 +438,9 @@ __global__ __launch_bounds__(256) void k_residual(const double2* __restrict__ X,
                                                    const double2* __restrict__ AX,
                                                    const double* __restrict__ lam,
                                                    const double* __restrict__ cts,
-                                                   float2* __restrict__ r32,
+                                                   void* __restrict__ rbuf,
                                                    double* __restrict__ nrm2, int n, int K,
-                                                   int rows_per_block, int first_col, int bw) {
+                                                   int rows_per_block, int first_col, int bw, int bf16) {
   const int ts = blockIdx.z;
   const int col = first_col + blockIdx.y * 64 + (threadIdx.x & 63);
   const int r0 = blockIdx.x * rows_per_block + (threadIdx.x >> 6);
@@ -317,8 +454,12 @@ __global__ __launch_bounds__(256) void k_residual(const double2* __restrict__ X,
     const double2 a = AX[idx], x = X[idx];
     const double rx = a.x - l * x.x, ry = a.y - l * x.y;
     acc += rx * rx + ry * ry;
-    r32[(((size_t)ts * (K / bw) + col / bw) * n + row) * bw + col % bw] =
-        make_float2((float)(c * rx), (float)(c * ry));
+    const size_t ri = (((size_t)ts * (K / bw) + col / bw) * n + row) * bw + col % bw;
+    const float2 rv = make_float2((float)(c * rx), (float)(c * ry));
+    if (bf16)
+      static_cast<unsigned*>(rbuf)[ri] = bf2_pack(rv);
+    else
+      static_cast<float2*>(rbuf)[ri] = rv;
   }
   __shared__ double part[256];
   part[threadIdx.x] = acc;
@@ -380,9 +521,20 @@ static inline dim3 site_grid(int L, int K, int nts) {
   return dim3((unsigned)((V3 + 3) / 4) * (unsigned)(K / 64) * (unsigned)nts);
 }
 
-hipError_t launch_filter_step32(const float2* U32, const float2* in, const float2* r, const float2* old,
-                                float2* out, const float4* coef, const float2* cab, const int* first_blk,
-                                int L, int K, int nts, int bw, hipStream_t st) {
+hipError_t launch_filter_step(const float2* U32, const void* in_, const void* r_, const void* old_, void* out_,
+                              const float4* coef, const float2* cab, const int* first_blk, int L, int K, int nts,
+                              int bw, int bf16, hipStream_t st) {
+  if (bf16) {
+    if (bw == 256)
+      launch_bf16<4>(U32, in_, r_, old_, out_, coef, cab, first_blk, L, K, nts, st);
+    else
+      launch_bf16<2>(U32, in_, r_, old_, out_, coef, cab, first_blk, L, K, nts, st);
+    return hipGetLastError();
+  }
+  const float2* in = static_cast<const float2*>(in_);
+  const float2* r = static_cast<const float2*>(r_);
+  const float2* old = static_cast<const float2*>(old_);
+  float2* out = static_cast<float2*>(out_);
   if (bw == 128) {
     const int V3 = L * L * L;
     dim3 g((unsigned)((V3 + 3) / 4) * (unsigned)(K / 128) * (unsigned)nts);
@@ -414,12 +566,13 @@ hipError_t launch_lap_block64(const double2* U64, const double2* v, double2* out
 }
 
 hipError_t launch_residual(const double2* X, const double2* AX, const double* lam, const double* cts,
-                           float2* r32, double* nrm2, int n, int K, int nts, int first_col, int bw, hipStream_t st) {
+                           void* r, double* nrm2, int n, int K, int nts, int first_col, int bw, int bf16,
+                           hipStream_t st) {
   const int w = K - first_col;
   k_zero_cols<<<(nts * w + 255) / 256, 256, 0, st>>>(nrm2, nts, K, first_col);
   const int rows = 256;
   dim3 g((n + rows - 1) / rows, w / 64, nts);
-  k_residual<<<g, 256, 0, st>>>(X, AX, lam, cts, r32, nrm2, n, K, rows, first_col, bw);
+  k_residual<<<g, 256, 0, st>>>(X, AX, lam, cts, r, nrm2, n, K, rows, first_col, bw, bf16);
   return hipGetLastError();
 }
 

@@ -52,7 +52,8 @@ class laph_eps_stats(ctypes.Structure):
 class laph_batch_opts(ctypes.Structure):
     _fields_ = [("nts", ctypes.c_int), ("nev", ctypes.c_int), ("nsub", ctypes.c_int),
                 ("tol", ctypes.c_double), ("max_iter", ctypes.c_int), ("seed", ctypes.c_ulonglong),
-                ("warm_start", ctypes.c_int), ("filter_passes", ctypes.c_int)]
+                ("warm_start", ctypes.c_int), ("filter_passes", ctypes.c_int),
+                ("filter_storage", ctypes.c_int)]
 
 
 class laph_batch_stats(ctypes.Structure):
@@ -304,11 +305,12 @@ class Batch:
     """All timeslices of a configuration in one mixed-precision batched solve."""
 
     def __init__(self, L, deg, lambda_l, lambda_c, nts, nev, nsub=0, tol=1e-10, max_iter=0, seed=1227,
-                 device=0, warm_start=0, filter_passes=0):
+                 device=0, warm_start=0, filter_passes=0, filter_storage=0):
         self.L, self.nts, self.nev = L, nts, nev
         self.n = 3 * L ** 3
         self.params = laph_params(L, deg, lambda_l, lambda_c, device)
-        self.opts = laph_batch_opts(nts, nev, nsub, tol, max_iter, seed, int(warm_start), int(filter_passes))
+        self.opts = laph_batch_opts(nts, nev, nsub, tol, max_iter, seed, int(warm_start), int(filter_passes),
+                                    int(filter_storage))
         self.h = ctypes.c_void_p()
         _check(lib().laph_batch_create(ctypes.byref(self.h), ctypes.byref(self.params),
                                        ctypes.byref(self.opts)), "laph_batch_create")

@@ -95,15 +95,16 @@ def test_batch_matches_plain_fp64(lp, gpu):
             assert np.abs(V[i] - Vpf[i]).max() < 1e-9, i
 
 
-@pytest.mark.parametrize("width", [64, 128])
-def test_filter_block_widths(lp, gpu, width, monkeypatch):
-    """Both layouts of the FP32 filter buffers (one vector per lane in
-    64-vector blocks, two per lane in 128-vector blocks) give the dense
-    oracle's spectrum, and the timing autotuner picks one of them."""
+@pytest.mark.parametrize("storage,width", [(0, 64), (0, 128), (1, 128), (1, 256)])
+def test_filter_block_widths(lp, gpu, storage, width, monkeypatch):
+    """Every layout of the filter buffers (FP32 storage: one or two vectors
+    per lane in 64- / 128-vector blocks; BF16 storage: two or four per lane
+    in 128- / 256-vector blocks) gives the dense oracle's spectrum, and the
+    timing autotuner picks one of the two widths of the storage format."""
     L, nts, nev, deg, tol = 8, 2, 60, 10, 1e-10
     data = smeared(L, 99, 0.35, nts)
     monkeypatch.setenv("LAPH_BW", str(width))
-    b = lp.Batch(L, deg, 12.0, 0.0, nts, nev, nsub=128, tol=tol, seed=5, device=gpu)
+    b = lp.Batch(L, deg, 12.0, 0.0, nts, nev, nsub=256, tol=tol, seed=5, device=gpu, filter_storage=storage)
     for t, (ts, _) in enumerate(data):
         b.set_timeslice(t, ts, 0.62, 0.62, 3)
     st = b.solve()
@@ -113,8 +114,25 @@ def test_filter_block_widths(lp, gpu, width, monkeypatch):
         check_pairs(ev, V, res, N.dense_D(U.reshape(L, L, L, 3, 3, 3)), nev, tol)
     b.close()
     monkeypatch.delenv("LAPH_BW")
-    b = lp.Batch(L, deg, 12.0, 0.0, nts, nev, nsub=128, tol=tol, seed=5, device=gpu)
+    b = lp.Batch(L, deg, 12.0, 0.0, nts, nev, nsub=256, tol=tol, seed=5, device=gpu, filter_storage=storage)
     for t, (ts, _) in enumerate(data):
         b.set_timeslice(t, ts, 0.62, 0.62, 3)
-    assert b.solve().filter_block in (64, 128)
+    assert b.solve().filter_block in ((64, 128) if storage == 0 else (128, 256))
+    b.close()
+
+
+@pytest.mark.parametrize("passes", [1, 2, 3, 4])
+def test_filter_passes(lp, gpu, passes):
+    """T_m(B)^P per cycle (correction pass + P-1 plain passes, each adding
+    gamma^(p-1) f_m in its last step) gives the dense oracle's spectrum."""
+    L, nts, nev, deg, tol = 8, 2, 40, 8, 1e-10
+    data = smeared(L, 31, 0.4, nts)
+    b = lp.Batch(L, deg, 12.0, 0.0, nts, nev, nsub=64, tol=tol, seed=2, device=gpu, filter_passes=passes)
+    for t, (ts, _) in enumerate(data):
+        b.set_timeslice(t, ts, 0.62, 0.62, 3)
+    st = b.solve()
+    assert st.converged == nts
+    for t, (_, U) in enumerate(data):
+        ev, V, ph, res = b.get(t)
+        check_pairs(ev, V, res, N.dense_D(U.reshape(L, L, L, 3, 3, 3)), nev, tol)
     b.close()
