@@ -4,15 +4,18 @@ mixed-precision batched solver vs the plain FP64 drop-in, on one MI355X.
 A step is one batched eigensolve of all 64 timeslices of the seeded
 synthetic configuration (U = exp(i 0.3 H), HYP 0.62/0.62/3 applied on the
 device beforehand; smeared links resident in HBM when the timed region
-starts): FP32 Chebyshev filter of degree 16 on 64-vector blocks, FP64
-CholQR2 / Rayleigh-Ritz / residual locking, until every timeslice has 220
-locked pairs with |D v - lambda v| <= 1e-10.
+starts): FP32 Chebyshev filter of degree 16 (T_16(B)^4 per cycle once the window has
+settled) on 64- or 128-vector blocks, FP64 CholQR2 / Rayleigh-Ritz / residual
+locking, until every timeslice has 220 locked pairs with
+|D v - lambda v| <= 1e-10.
 
 Printed JSON (one line) also carries:
   roofline      the FP32 filter stencil (dominant kernel): algorithmic bytes
                 per launch / average launch duration, from a replay of the
                 captured filter alone on the device (HIP events on the
-                library's stream), vs 8 TB/s
+                library's stream), vs 8 TB/s; `traffic` = measured HBM bytes
+                per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE on those
+                replays, each scaled by the same counter on a 1 GiB copy)
   plain_fp64    the plain FP64 drop-in (device-resident Krylov-Schur on
                 T_16(B), lambda_c = 1.35 |lambda_220|) timed on a few
                 timeslices of the same configuration, timeslices/s
@@ -35,9 +38,12 @@ L, LT, EPS, NEV, DEG, TOL, NSUB = 32, 64, 0.3, 220, 16, 1e-10, 256
 SEED = 1227
 LAMBDA_L = 12.0
 PEAK_HBM = 8.0e12
-# HBM bytes per k_filter_step32 launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-# (scaled by the same counters on a copy of known size); None until measured.
-TRAFFIC_GB_PER_LAUNCH = None
+# HBM bytes per filter-step launch (all columns active, 32 timeslices of one
+# lane, 128-vector blocks) from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the
+# isolated replays of --profile-filter, each scaled by the same counter on a
+# 1 GiB device copy (python bench.py --summarize-pmc ...); None = not measured
+# for this build.
+TRAFFIC_BYTES_PER_LAUNCH = 38821704728  # profiles/filter_step_32x32x32x64_k256_pmc.md
 
 
 def main():
@@ -51,7 +57,19 @@ def main():
     ap.add_argument("--filter-reps", type=int, default=20)
     ap.add_argument("--profile-filter", action="store_true",
                     help="counter-collection workload: filter replays + a calibration copy, no solve timing")
+    ap.add_argument("--summarize-trace", metavar="DB",
+                    help="per-kernel durations from a rocprofv3 --kernel-trace database (no GPU use)")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="HBM bytes per filter launch from two rocprofv3 --pmc runs of --profile-filter")
+    ap.add_argument("--last", type=int, default=0,
+                    help="with --summarize-*: the isolated filter replays are the last LAST filter dispatches")
     args = ap.parse_args()
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace, args.last)
+        return
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc, args.last)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); timeslices are "
               "independent, so N GPUs would each run their own Lt/N share with no exchange.", file=sys.stderr)
@@ -100,8 +118,9 @@ def main():
     b.close()
     achieved = bytes_launch / (ms_launch * 1e-3)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                "frac": achieved / PEAK_HBM, "traffic": TRAFFIC_GB_PER_LAUNCH,
-                "kernel": "k_filter_step32", "avg_launch_us": ms_launch * 1e3, "bytes_per_launch": bytes_launch,
+                "frac": achieved / PEAK_HBM, "traffic": TRAFFIC_BYTES_PER_LAUNCH, "traffic_unit": "bytes/launch",
+                "kernel": f"k_filter_step32{'x2' if st.filter_block == 128 else ''} (block {st.filter_block})",
+                "avg_launch_us": ms_launch * 1e3, "bytes_per_launch": bytes_launch,
                 "in_solve_GBps": tot["fbytes"] / tot["filter"] / 1e9 if tot["filter"] > 0 else None}
 
     # plain FP64 drop-in on a few timeslices of the same field
@@ -127,7 +146,8 @@ def main():
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32 filter + fp64 CholQR2/RR/lock", "data": "synthetic",
         "config": {"workload": f"{L}^3x{nts} eps={EPS} HYP 0.62/0.62/3, N_ev={NEV}, Chebyshev degree {DEG}, "
-                               f"vector block 64, search space {NSUB}, residual <= {TOL}",
+                               f"T_{DEG}(B)^4 per cycle, vector block {st.filter_block}, search space {NSUB}, "
+                               f"residual <= {TOL}",
                    "lattice": f"{L}^3x{nts}", "nev": NEV, "deg": DEG, "tol": TOL, "parallelism": "none (1 GPU)"},
         "roofline": roofline,
         "plain_fp64": {"value": plain_value, "unit": "timeslices/s", "timeslices": len(plain_times),
@@ -165,7 +185,64 @@ def profile_filter(lp, args):
     hip.hipFree(src)
     hip.hipFree(dst)
     print(json.dumps({"filter_ms_per_launch": ms, "filter_bytes_per_launch": by, "filter_GBps": by / ms / 1e6,
-                      "lane_timeslices": (nts + 1) // 2, "copy_bytes": nbytes}))
+                      "lane_timeslices": nts - nts // 2, "copy_bytes": nbytes,
+                      "isolated_filter_dispatches": (args.filter_reps + 1) * (DEG - 1)}))
+
+
+def _filter_kernel(name):
+    return "k_filter_step" in name
+
+
+def summarize_trace(db, last):
+    """Per-kernel totals of a rocprofv3 kernel-trace database, and the mean
+    duration of the last `last` filter dispatches (the isolated replays)."""
+    import sqlite3
+    c = sqlite3.connect(db)
+    rows = list(c.execute(
+        "select s.kernel_name, d.start, d.end from rocpd_kernel_dispatch d "
+        "join rocpd_info_kernel_symbol s on d.kernel_id = s.id order by d.start"))
+    agg = {}
+    for name, t0, t1 in rows:
+        a = agg.setdefault(name, [0, 0.0, float("inf"), 0.0])
+        dt = (t1 - t0) / 1e3
+        a[0] += 1
+        a[1] += dt
+        a[2] = min(a[2], dt)
+        a[3] = max(a[3], dt)
+    tot = sum(a[1] for a in agg.values())
+    print(f"{'kernel':72s} {'calls':>7s} {'total_ms':>10s} {'avg_us':>10s} {'min_us':>9s} {'max_us':>9s} {'pct':>6s}")
+    for name, a in sorted(agg.items(), key=lambda kv: -kv[1][1])[:25]:
+        print(f"{name[:72]:72s} {a[0]:7d} {a[1] / 1e3:10.2f} {a[1] / a[0]:10.2f} {a[2]:9.2f} {a[3]:9.2f} "
+              f"{100 * a[1] / tot:6.1f}")
+    if last:
+        fl = [(t1 - t0) / 1e3 for name, t0, t1 in rows if _filter_kernel(name)][-last:]
+        print(f"isolated filter replays: last {len(fl)} filter dispatches, mean {sum(fl) / len(fl):.2f} us")
+
+
+def summarize_pmc(fetch_csv, write_csv, last):
+    """HBM bytes per isolated filter launch: mean FETCH_SIZE and WRITE_SIZE
+    (KiB per dispatch) over the last `last` filter dispatches, each scaled by
+    the known bytes of the 1 GiB calibration copy over what the same counter
+    reads for it (the copy dispatches after the last filter dispatch)."""
+    import csv
+    nbytes = 1 << 30
+
+    def read(path, counter):
+        rows = [r for r in csv.DictReader(open(path)) if r["Counter_Name"] == counter]
+        rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+        filt = [i for i, r in enumerate(rows) if _filter_kernel(r["Kernel_Name"])]
+        sel = filt[-last:]
+        val = sum(float(rows[i]["Counter_Value"]) for i in sel) / len(sel)
+        copy = sum(float(r["Counter_Value"]) for r in rows[filt[-1] + 1:] if "copyBuffer" in r["Kernel_Name"])
+        return val, copy, len(sel)
+
+    f, fc, nf = read(fetch_csv, "FETCH_SIZE")
+    w, wc, nw = read(write_csv, "WRITE_SIZE")
+    kf, kw = (nbytes / 1024) / fc, (nbytes / 1024) / wc
+    traffic = (f * kf + w * kw) * 1024
+    print(json.dumps({"filter_dispatches": [nf, nw], "fetch_KiB": f, "write_KiB": w,
+                      "copy_fetch_KiB": fc, "copy_write_KiB": wc, "fetch_scale": kf, "write_scale": kw,
+                      "traffic_bytes_per_launch": traffic}))
 
 
 def device_sync():

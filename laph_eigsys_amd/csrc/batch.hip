@@ -222,18 +222,41 @@ static int project_locked(const View& v, Act q) {
 }
 
 // Explicit CholQR of the active block: Yc_A <- U^{-H} Yc_A, Yc_A Yc_A^H = U^H U.
-static int cholqr(const View& v, Act q) {
+// G_A += shift * I for every timeslice of the lane (a x a block, lda K).
+__global__ void k_shift_diag(double2* __restrict__ G, int a, int K, double shift) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a) G[(size_t)blockIdx.y * K * K + (size_t)i * K + i].x += shift;
+}
+
+// One Cholesky QR pass of the active block: Y_A <- Y_A R^{-1}, R^H R = Y_A^H Y_A.
+// When the Gram is numerically singular (high-degree filtering can make
+// buffer columns nearly parallel to the dominant eigenvectors), the pass is
+// done on the shifted Gram Y_A^H Y_A + s I, s = 11 (n a + a (a+1)) u |Y_A|_2^2
+// (shifted Cholesky QR: the Cholesky then exists and cond(Y_A R^{-1}) is
+// O(s^{-1/2})), followed by a plain pass.
+static int cholqr(const View& v, Act q, bool shifted = false) {
   const int K = v.b->K;
   const cplx one(1.0, 0.0);
   double2* Y = v.X();
   double2* G = v.b->G + v.kk();
-  // unit columns first: the filtered columns of the first cycles differ in
-  // norm by many orders of magnitude (the span is unchanged by the scaling)
+  if (!shifted && v.b->force_shift) LAPH_TRY(cholqr(v, q, true));  // LAPH_SHIFTED_CHOLQR (tests)
+  // unit columns first: the filtered columns differ in norm by many orders
+  // of magnitude (the span is unchanged by the scaling); then |Y_A|_2^2 <= a
   LAPH_HIP(launch_normalize_columns(Y, v.b->cn + v.nk(), v.b->n, K, v.ln->nt, q.l, v.ln->st));
   LAPH_TRY(gram(v, q, Y, Y, G, 1.0));
+  if (shifted) {
+    const double a = q.a, u = 0x1p-53;
+    const double sh = 11.0 * ((double)v.b->n * a + a * (a + 1.0)) * u * a;
+    k_shift_diag<<<dim3((q.a + 255) / 256, v.ln->nt), 256, 0, v.ln->st>>>(G, q.a, K, sh);
+    LAPH_HIP(hipGetLastError());
+  }
   LAPH_RB(rocsolver_zpotrf_strided_batched(v.ln->rb, rocblas_fill_upper, q.a, zc(G), K, (rocblas_stride)K * K,
                                            v.b->info + v.ln->t0, v.ln->nt));
-  LAPH_TRY(dense_sync_info(v, "CholQR (zpotrf)"));
+  if (dense_sync_info(v, "CholQR (zpotrf)") != LAPH_OK) {
+    if (shifted) return LAPH_ERR_NOCONV;  // the message names the failing timeslice
+    LAPH_TRY(cholqr(v, q, true));
+    return cholqr(v, q);
+  }
   LAPH_RB(rocblas_ztrsm_strided_batched(v.ln->rb, rocblas_side_left, rocblas_fill_upper,
                                         rocblas_operation_conjugate_transpose, rocblas_diagonal_non_unit, q.a, v.b->n,
                                         &one, zc(G), K, (rocblas_stride)K * K, zc(Y + q.l), K,
@@ -569,10 +592,22 @@ __global__ void k_combine_gamma(double2* __restrict__ X, const E* __restrict__ W
   X[idx] = make_double2(g * x.x + (double)w.x, g * x.y + (double)w.y);
 }
 
-// Applications of T_m(B) per cycle once the window has settled.
-static int filter_passes(const laph_batch* b) {
-  const int p = b->o.filter_passes > 0 ? b->o.filter_passes : 4;
-  return std::min(p, LAPH_MAX_PASSES);
+// Applications of T_m(B) in this cycle of a lane: the requested count, cut
+// so that the largest amplification T_m(theta_max)^P stays <= 1e8, with
+// theta_max = 1 + c lambda_c >= the image of the lowest eigenvalue of -D
+// (-D >= 0).  A column whose Ritz value sits at the window edge (theta = 1,
+// no scaling) grows its components along the lowest eigenvectors by that
+// factor: beyond ~1e8 such buffer columns become numerically parallel in
+// FP64, and a wide early window would overflow FP32.
+static int filter_passes(const laph_batch* b, const std::vector<double>& lc) {
+  const int want = std::min(b->o.filter_passes > 0 ? b->o.filter_passes : 4, LAPH_MAX_PASSES);
+  double amax = 1.0;
+  for (double l : lc) {
+    const double th = 1.0 + 2.0 * l / (b->p.lambda_l - l);
+    amax = std::max(amax, std::cosh(b->deg * std::acosh(th)));
+  }
+  const int pmax = amax > 1.0 ? (int)std::floor(std::log(1e8) / std::log(amax)) : want;
+  return std::max(1, std::min(want, pmax));
 }
 
 // Solve loop of one lane (runs on its own host thread).
@@ -601,7 +636,7 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
   int it = 0;
   for (it = 0; it < max_iter && active > 0; ++it) {
     // one filter pass while the window settles, then T_m(B)^P per cycle
-    const int passes = it < b->warm_cycles ? 1 : filter_passes(b);
+    const int passes = it < b->warm_cycles ? 1 : filter_passes(b, lc);
     LAPH_TRY(upload_coefficients(v, lc, passes));
     for (int t = 0; t < nt; ++t) {
       const int nl = b->nlock[ln->t0 + t];
@@ -801,6 +836,7 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     b->per_ts = (size_t)n * K;
     b->trace = std::getenv("LAPH_TRACE") != nullptr;
     if (const char* e = std::getenv("LAPH_WARM")) b->warm_cycles = std::max(0, std::atoi(e));
+    b->force_shift = std::getenv("LAPH_SHIFTED_CHOLQR") != nullptr;
     b->bf16 = o->filter_storage == 1;
     b->esz = b->bf16 ? 4 : 8;
     if (const char* e = std::getenv("LAPH_EIG")) b->eig_mode = std::atoi(e);

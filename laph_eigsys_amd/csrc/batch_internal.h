@@ -88,5 +88,6 @@ struct laph_batch {
   int bw = 0;
   int bf16 = 0;  // filter vectors stored as BF16 (laph_batch_opts.filter_storage)
   int warm_cycles = 2;  // single-pass cycles while the window settles (LAPH_WARM)
+  bool force_shift = false;  // every CholQR pass starts with a shifted one (LAPH_SHIFTED_CHOLQR)
 };
 

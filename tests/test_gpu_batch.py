@@ -136,3 +136,19 @@ def test_filter_passes(lp, gpu, passes):
         ev, V, ph, res = b.get(t)
         check_pairs(ev, V, res, N.dense_D(U.reshape(L, L, L, 3, 3, 3)), nev, tol)
     b.close()
+
+
+def test_shifted_cholqr_path(lp, gpu, monkeypatch):
+    """Every Cholesky QR pass preceded by the shifted pass (the fallback for
+    numerically singular Grams): same spectrum as the dense oracle."""
+    L, nts, nev, deg, tol = 8, 2, 40, 8, 1e-10
+    data = smeared(L, 57, 0.4, nts)
+    monkeypatch.setenv("LAPH_SHIFTED_CHOLQR", "1")
+    b = lp.Batch(L, deg, 12.0, 0.0, nts, nev, nsub=64, tol=tol, seed=4, device=gpu)
+    for t, (ts, _) in enumerate(data):
+        b.set_timeslice(t, ts, 0.62, 0.62, 3)
+    assert b.solve().converged == nts
+    for t, (_, U) in enumerate(data):
+        ev, V, ph, res = b.get(t)
+        check_pairs(ev, V, res, N.dense_D(U.reshape(L, L, L, 3, 3, 3)), nev, tol)
+    b.close()
