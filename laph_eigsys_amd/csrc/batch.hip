@@ -316,7 +316,7 @@ static int rayleigh_ritz(const View& v, Act q, bool orthonormal, bool* chol_fail
   LAPH_RB(rocblas_zgemm_strided_batched(rb, rocblas_operation_conjugate_transpose, rocblas_operation_none, q.a, b->n,
                                         q.a, &one, zc(G), K, kk, zc(Y + q.l), K, (rocblas_stride)b->per_ts, &zero,
                                         zc(T + q.l), K, (rocblas_stride)b->per_ts, nt));
-  LAPH_HIP(launch_lap_block64(b->U64 + v.lo(), T, AY, b->p.L, K, nt, b->lock_blk + v.ln->t0, v.ln->st));
+  LAPH_HIP(launch_lap_block64(b->U64 + v.lo(), T, AY, b->p.L, K, nt, b->lock_blk + v.ln->t0, b->strip, v.ln->st));
   // the hard-locked columns of X live in the old X buffer: carry them over
   if (q.l > 0)
     LAPH_HIP(hipMemcpy2DAsync(T, (size_t)K * sizeof(double2), Y, (size_t)K * sizeof(double2),
@@ -339,7 +339,7 @@ static int orthonormalise_and_rr(const View& v, Act q, bool explicit_qr) {
       LAPH_TRY(cholqr(v, q));
     }
     LAPH_HIP(launch_lap_block64(b->U64 + v.lo(), v.X(), v.AX(), b->p.L, b->K, v.ln->nt, b->lock_blk + v.ln->t0,
-                                v.ln->st));
+                                b->strip, v.ln->st));
     v.ln->sv64 += 2LL * v.ln->nt * q.a;  // D Y for Rayleigh-Ritz, D X after it
     bool failed = false;
     LAPH_TRY(rayleigh_ritz(v, q, explicit_qr || pass == 1, &failed));
@@ -472,7 +472,7 @@ static int enqueue_filter(const View& v) {
     const void* old = st >= 3 ? fb(b, b->P[(st - 3) % 3], vo) : nullptr;
     void* out = fb(b, b->P[(st - 1) % 3], vo);
     LAPH_HIP(launch_filter_step(b->U32 + v.lo(), in, R, old, out, b->coef + (st - 1) * step + v.nk(),
-                                b->cab + v.ln->t0, b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, v.ln->st));
+                                b->cab + v.ln->t0, b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, b->strip, v.ln->st));
   }
   return LAPH_OK;
 }
@@ -524,7 +524,7 @@ static int enqueue_plain_pass(const View& v, int p) {
     const bool last = st == m;
     const float4* co = b->coef2 + (size_t)(last ? m - 1 + p - 2 : st - 1) * step + v.nk();
     LAPH_HIP(launch_filter_step(b->U32 + v.lo(), in, last ? f : nullptr, old, out, co, b->cab + v.ln->t0,
-                                b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, v.ln->st));
+                                b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, b->strip, v.ln->st));
   }
   return LAPH_OK;
 }
@@ -721,36 +721,53 @@ static int time_correction_pass(laph_batch* b, int reps, float* ms) {
   return LAPH_OK;
 }
 
-// Block width of the FP32 filter buffers: both widths timed on lane 0's
-// timeslices with every column active (the kernels only stream the buffers;
-// their contents do not change the time), the faster kept.
+// Layout of the filter: block width (two candidates of the storage format)
+// and sweep strip width (L, or 4 when it divides L), each pair timed on
+// lane 0's timeslices with every column active (the kernels only stream the
+// buffers; their contents do not change the time); the fastest is kept.
+// LAPH_BW / LAPH_STRIP fix either.
 static int autotune_block_width(laph_batch* b) {
   laph_lane* ln = &b->lanes[0];
   LAPH_HIP(hipMemsetAsync(b->first_blk + ln->t0, 0, ln->nt * sizeof(int), ln->st));
   int cand[2];
   block_widths(b, cand);
-  float best = 0;
-  int wbest = cand[0];
-  for (int w : cand) {
-    if (b->K % w != 0) continue;
-    b->bw = w;
-    float ms = 0;
-    LAPH_TRY(time_correction_pass(b, 3, &ms));
-    if (w == cand[0] || ms < best) {
-      best = ms;
-      wbest = w;
-    }
+  std::vector<int> widths, strips;
+  for (int w : cand)
+    if (b->K % w == 0 && (b->bw == 0 || b->bw == w)) widths.push_back(w);
+  if (b->strip_fixed) {
+    strips.push_back(b->strip);
+  } else {
+    strips.push_back(b->p.L);
+    if (b->p.L % 4 == 0 && b->p.L > 4) strips.push_back(4);
   }
+  float best = 0;
+  int wbest = widths[0], sbest = strips[0];
+  bool first = true;
+  for (int w : widths)
+    for (int sw : strips) {
+      b->bw = w;
+      b->strip = sw;
+      float ms = 0;
+      LAPH_TRY(time_correction_pass(b, 2, &ms));
+      if (first || ms < best) {
+        best = ms;
+        wbest = w;
+        sbest = sw;
+        first = false;
+      }
+    }
   b->bw = wbest;
-  LAPH_HIP(hipGraphExecDestroy(ln->graph));  // captured for the last width timed
+  b->strip = sbest;
+  b->tuned = true;
+  LAPH_HIP(hipGraphExecDestroy(ln->graph));  // captured for the last layout timed
   ln->graph = nullptr;
-  if (b->trace) std::fprintf(stderr, "[laph] filter block width %d\n", wbest);
+  if (b->trace) std::fprintf(stderr, "[laph] filter block width %d, strip %d\n", wbest, sbest);
   return LAPH_OK;
 }
 
 static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
   auto t0 = std::chrono::steady_clock::now();
-  if (b->bw == 0) LAPH_TRY(autotune_block_width(b));
+  if (!b->tuned) LAPH_TRY(autotune_block_width(b));
   const bool warm = b->o.warm_start && b->have_start;
   b->have_start = false;
   b->solved = false;
@@ -789,6 +806,7 @@ static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
   }
   s.iterations = cycles;
   s.filter_block = b->bw;
+  s.filter_strip = b->strip;
   s.converged = conv;
   s.max_residual = maxres;
   s.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -823,6 +841,11 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
   int ndev = 0;
   LAPH_HIP(hipGetDeviceCount(&ndev));
   LAPH_CHECK(p->device >= 0 && p->device < ndev, LAPH_ERR_INVALID, "laph_batch_create: no device %d", p->device);
+  int strip = 0;  // LAPH_STRIP: fixed sweep strip width
+  if (const char* e = std::getenv("LAPH_STRIP")) {
+    strip = std::atoi(e);
+    LAPH_CHECK(strip >= 1 && p->L % strip == 0, LAPH_ERR_INVALID, "LAPH_STRIP=%s must divide L=%d", e, p->L);
+  }
   return laph::guarded([&]() -> int {
     laph_batch* b = new laph_batch();
     b->p = *p;
@@ -837,6 +860,8 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     b->trace = std::getenv("LAPH_TRACE") != nullptr;
     if (const char* e = std::getenv("LAPH_WARM")) b->warm_cycles = std::max(0, std::atoi(e));
     b->force_shift = std::getenv("LAPH_SHIFTED_CHOLQR") != nullptr;
+    b->strip = strip > 0 ? strip : p->L;
+    b->strip_fixed = strip > 0;
     b->bf16 = o->filter_storage == 1;
     b->esz = b->bf16 ? 4 : 8;
     if (const char* e = std::getenv("LAPH_EIG")) b->eig_mode = std::atoi(e);

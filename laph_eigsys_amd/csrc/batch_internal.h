@@ -83,11 +83,14 @@ struct laph_batch {
   rocblas_int* jsweeps = nullptr;
   bool trace = false;       // LAPH_TRACE set: per-cycle trace of timeslice 0 on stderr
   bool have_start = false;  // B[0] holds a proposed start block (laph_net_propose)
-  // vectors per block of the FP32 filter buffers (64 or 128); 0 until the
-  // first solve times both widths (LAPH_BW=64|128 fixes it)
+  // vectors per block of the filter buffers; 0 until the first solve times
+  // the candidates (LAPH_BW fixes it)
   int bw = 0;
   int bf16 = 0;  // filter vectors stored as BF16 (laph_batch_opts.filter_storage)
   int warm_cycles = 2;  // single-pass cycles while the window settles (LAPH_WARM)
+  int strip = 0;  // sweep strip width of the stencil kernels (divides L; L = x-major planes)
+  bool strip_fixed = false;  // LAPH_STRIP given
+  bool tuned = false;        // filter layout chosen (first solve)
   bool force_shift = false;  // every CholQR pass starts with a shifted one (LAPH_SHIFTED_CHOLQR)
 };
 

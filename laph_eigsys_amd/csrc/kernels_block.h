@@ -15,10 +15,11 @@ namespace laph {
 // two or four per lane); buffers block-major with bw-vector blocks.
 hipError_t launch_filter_step(const float2* U32, const void* in, const void* r, const void* old, void* out,
                               const float4* coef, const float2* cab, const int* first_blk, int L, int K, int nts,
-                              int bw, int bf16, hipStream_t st);
+                              int bw, int bf16, int W, hipStream_t st);
 // out = D v on every (or, with first_blk, every active) column.
+// W: sweep strip width (divides L; L = plain x-major order)
 hipError_t launch_lap_block64(const double2* U64, const double2* v, double2* out, int L, int K, int nts,
-                              const int* first_blk, hipStream_t st);
+                              const int* first_blk, int W, hipStream_t st);
 // R = AX - X diag(lam), r32 = c_t R, nrm2 = column sums of |R|^2, for the
 // columns >= first_col (a multiple of 64).
 hipError_t launch_residual(const double2* X, const double2* AX, const double* lam, const double* cts,

@@ -45,6 +45,21 @@ __device__ __forceinline__ void unit_map(int SB, int nb, int nts, int& ts, int& 
   cb = (int)(unit % nb);
 }
 
+// Site of sweep index i when each unit is swept in y-strips of width W:
+// i = ((strip * L + x) * W + yy) * L + z, y = strip * W + yy (W divides L;
+// W = L is plain x-major order).  With narrow strips the x-neighbour planes
+// of the sites in flight on an XCD (W * L sites each) stay in its L2.
+__device__ __forceinline__ int strip_site(int i, int L, int W) {
+  const int LW = L * W;
+  const int strip = i / (LW * L);
+  const int r = i - strip * (LW * L);
+  const int x = r / LW;
+  const int r2 = r - x * LW;
+  const int yy = r2 / L;
+  const int z = r2 - yy * L;
+  return (x * L + strip * W + yy) * L + z;
+}
+
 typedef float f2v __attribute__((ext_vector_type(2)));
 typedef double d2v __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ float2 ld_nt(const float2* p) {
@@ -140,14 +155,15 @@ __global__ __launch_bounds__(256) void k_filter_step32(const float2* __restrict_
                                                         const float4* __restrict__ coef,
                                                         const float2* __restrict__ cab,
                                                         const int* __restrict__ first_blk, int L,
-                                                        int K, int nts, int use_old) {
+                                                        int K, int nts, int use_old, int W) {
   const int V3 = L * L * L;
   int ts, cb, sb;
   unit_map((V3 + 3) / 4, K / 64, nts, ts, cb, sb);
   if (cb < first_blk[ts]) return;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int s = sb * 4 + wave;
-  if (s >= V3) return;
+  const int si = sb * 4 + wave;
+  if (si >= V3) return;
+  const int s = strip_site(si, L, W);
   const int l64 = threadIdx.x & 63;
   const int lane = cb * 64 + l64;
   // FP32 filter buffers are block-major: [ts][cb][3*V3][64]
@@ -192,14 +208,15 @@ __global__ __launch_bounds__(256) void k_filter_step32x2(const float2* __restric
                                                           const float4* __restrict__ coef,
                                                           const float2* __restrict__ cab,
                                                           const int* __restrict__ first_blk, int L,
-                                                          int K, int nts, int use_old) {
+                                                          int K, int nts, int use_old, int W) {
   const int V3 = L * L * L;
   int ts, cb, sb;
   unit_map((V3 + 3) / 4, K / 128, nts, ts, cb, sb);
   if (2 * cb + 1 < first_blk[ts]) return;  // both 64-vector halves inactive
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int s = sb * 4 + wave;
-  if (s >= V3) return;
+  const int si = sb * 4 + wave;
+  if (si >= V3) return;
+  const int s = strip_site(si, L, W);
   const int l64 = threadIdx.x & 63;
   const int col = cb * 128 + 2 * l64;
   if (col < first_blk[ts] * 64) return;  // lanes of a locked 64-vector half issue no traffic
@@ -303,7 +320,7 @@ __global__ __launch_bounds__(256) void k_filter_step_bf16(const float2* __restri
                                                            const float4* __restrict__ coef,
                                                            const float2* __restrict__ cab,
                                                            const int* __restrict__ first_blk, int L, int K,
-                                                           int nts, int use_old) {
+                                                           int nts, int use_old, int W) {
   typedef typename UVec<VPL>::T T;
   constexpr int BW = 64 * VPL;
   const int V3 = L * L * L;
@@ -311,8 +328,9 @@ __global__ __launch_bounds__(256) void k_filter_step_bf16(const float2* __restri
   unit_map((V3 + 3) / 4, K / BW, nts, ts, cb, sb);
   if ((cb + 1) * VPL <= first_blk[ts]) return;  // every 64-vector part of the block locked
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int s = sb * 4 + wave;
-  if (s >= V3) return;
+  const int si = sb * 4 + wave;
+  if (si >= V3) return;
+  const int s = strip_site(si, L, W);
   const int l64 = threadIdx.x & 63;
   const int col = cb * BW + VPL * l64;
   if (col < first_blk[ts] * 64) return;  // lanes of locked 64-vector parts issue no traffic
@@ -390,7 +408,7 @@ __global__ __launch_bounds__(256) void k_filter_step_bf16(const float2* __restri
 template <int VPL>
 static void launch_bf16(const float2* U32, const void* in, const void* r, const void* old, void* out,
                         const float4* coef, const float2* cab, const int* first_blk, int L, int K, int nts,
-                        hipStream_t st) {
+                        int W, hipStream_t st) {
   typedef typename UVec<VPL>::T T;
   const int V3 = L * L * L;
   dim3 g((unsigned)((V3 + 3) / 4) * (unsigned)(K / (64 * VPL)) * (unsigned)nts);
@@ -400,24 +418,25 @@ static void launch_bf16(const float2* U32, const void* in, const void* r, const 
   T* out4 = static_cast<T*>(out);
   if (r)
     k_filter_step_bf16<true, VPL><<<g, 256, 0, st>>>(U32, i4, r4, o4, out4, coef, cab, first_blk, L, K, nts,
-                                                     old != nullptr);
+                                                     old != nullptr, W);
   else
     k_filter_step_bf16<false, VPL><<<g, 256, 0, st>>>(U32, i4, r4, o4, out4, coef, cab, first_blk, L, K, nts,
-                                                      old != nullptr);
+                                                      old != nullptr, W);
 }
 
 // FP64 Laplacian on a block: out = H v - 6 v (= D v) for all columns.
 __global__ __launch_bounds__(256) void k_lap_block64(const double2* __restrict__ U64,
                                                       const double2* __restrict__ v,
                                                       double2* __restrict__ out, int L, int K, int nts,
-                                                      const int* __restrict__ first_blk) {
+                                                      const int* __restrict__ first_blk, int W) {
   const int V3 = L * L * L;
   int ts, cb, sb;
   unit_map((V3 + 3) / 4, K / 64, nts, ts, cb, sb);
   if (first_blk && cb < first_blk[ts]) return;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int s = sb * 4 + wave;
-  if (s >= V3) return;
+  const int si = sb * 4 + wave;
+  if (si >= V3) return;
+  const int s = strip_site(si, L, W);
   const int lane = cb * 64 + (threadIdx.x & 63);
   const size_t vo = (size_t)ts * V3 * 3 * K;
   double2 acc[3], ctr[3];
@@ -523,12 +542,12 @@ static inline dim3 site_grid(int L, int K, int nts) {
 
 hipError_t launch_filter_step(const float2* U32, const void* in_, const void* r_, const void* old_, void* out_,
                               const float4* coef, const float2* cab, const int* first_blk, int L, int K, int nts,
-                              int bw, int bf16, hipStream_t st) {
+                              int bw, int bf16, int W, hipStream_t st) {
   if (bf16) {
     if (bw == 256)
-      launch_bf16<4>(U32, in_, r_, old_, out_, coef, cab, first_blk, L, K, nts, st);
+      launch_bf16<4>(U32, in_, r_, old_, out_, coef, cab, first_blk, L, K, nts, W, st);
     else
-      launch_bf16<2>(U32, in_, r_, old_, out_, coef, cab, first_blk, L, K, nts, st);
+      launch_bf16<2>(U32, in_, r_, old_, out_, coef, cab, first_blk, L, K, nts, W, st);
     return hipGetLastError();
   }
   const float2* in = static_cast<const float2*>(in_);
@@ -544,24 +563,24 @@ hipError_t launch_filter_step(const float2* U32, const void* in_, const void* r_
     float4* out4 = reinterpret_cast<float4*>(out);
     if (r)
       k_filter_step32x2<true><<<g, 256, 0, st>>>(U32, in4, r4, o4, out4, coef, cab, first_blk, L, K, nts,
-                                                 old != nullptr);
+                                                 old != nullptr, W);
     else
       k_filter_step32x2<false><<<g, 256, 0, st>>>(U32, in4, r4, o4, out4, coef, cab, first_blk, L, K, nts,
-                                                  old != nullptr);
+                                                  old != nullptr, W);
     return hipGetLastError();
   }
   if (r)
     k_filter_step32<true><<<site_grid(L, K, nts), 256, 0, st>>>(U32, in, r, old, out, coef, cab, first_blk, L, K, nts,
-                                                              old != nullptr);
+                                                              old != nullptr, W);
   else
     k_filter_step32<false><<<site_grid(L, K, nts), 256, 0, st>>>(U32, in, r, old, out, coef, cab, first_blk, L, K,
-                                                               nts, old != nullptr);
+                                                               nts, old != nullptr, W);
   return hipGetLastError();
 }
 
 hipError_t launch_lap_block64(const double2* U64, const double2* v, double2* out, int L, int K, int nts,
-                              const int* first_blk, hipStream_t st) {
-  k_lap_block64<<<site_grid(L, K, nts), 256, 0, st>>>(U64, v, out, L, K, nts, first_blk);
+                              const int* first_blk, int W, hipStream_t st) {
+  k_lap_block64<<<site_grid(L, K, nts), 256, 0, st>>>(U64, v, out, L, K, nts, first_blk, W);
   return hipGetLastError();
 }
 

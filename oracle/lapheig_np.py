@@ -189,6 +189,31 @@ def dense_D(U):
     return D
 
 
+def sparse_minus_D(U):
+    """-D = 6 - H as a scipy CSR matrix assembled entry by entry from the
+    links (config_utils.cpp:18-58: diagonal 6; row 3x+a, column 3(x+mu)+b:
+    -U_mu(x)[a,b]; column 3(x-mu)+b: -U_mu(x-mu)^H[a,b]); site x = (x*L+y)*L+z,
+    periodic neighbours (navigation.cpp:27-88)."""
+    import scipy.sparse as sp
+    L = U.shape[0]
+    V3 = L ** 3
+    idx = np.arange(V3).reshape(L, L, L)
+    rows, cols, vals = [np.arange(3 * V3)], [np.arange(3 * V3)], [np.full(3 * V3, 6.0 + 0j)]
+    a = np.arange(3)
+    for mu in range(3):
+        up = shift(idx, mu, 1).reshape(-1)
+        dn = shift(idx, mu, -1).reshape(-1)
+        Um = U[..., mu, :, :].reshape(V3, 3, 3)
+        Udn = Um[dn]  # U_mu(x - mu)
+        r = (3 * np.arange(V3)[:, None, None] + a[None, :, None]) * np.ones((1, 1, 3), dtype=np.int64)
+        rows += [r.reshape(-1), r.reshape(-1)]
+        cols += [(3 * up[:, None, None] + a[None, None, :] + 0 * r).reshape(-1),
+                 (3 * dn[:, None, None] + a[None, None, :] + 0 * r).reshape(-1)]
+        vals += [(-Um).reshape(-1), (-np.conj(np.swapaxes(Udn, 1, 2))).reshape(-1)]
+    n = 3 * V3
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+
+
 def cheb_filter(U, psi, lamL, lamC, deg):
     c = 2.0 / (lamL - lamC)
     a = 1.0 + c * lamC

@@ -256,3 +256,15 @@ def test_eps_solve_against_dense_eigh(L, nev, deg, tol):
         r = np.linalg.norm(D @ V[i] - ev[i] * V[i])
         assert r < 1e-9
     assert O.check_trace(V)[0]
+
+
+def test_sparse_assembly_matches_stencil():
+    """The entry-by-entry sparse -D (BuildLaplacian layout, source of the C2
+    fixture) equals the numpy stencil, and is Hermitian."""
+    L = 5
+    U = N.hyp(N.links_from_config(N.synth_config(3, 0.4, L, 1), L), 0.62, 0.62, 1)
+    A = N.sparse_minus_D(U)
+    rng = np.random.default_rng(0)
+    v = rng.normal(size=(L, L, L, 3)) + 1j * rng.normal(size=(L, L, L, 3))
+    assert np.abs(A @ v.reshape(-1) + N.apply_D(U, v).reshape(-1)).max() < 1e-13
+    assert abs(A - A.conj().T).max() == 0
