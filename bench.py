@@ -61,6 +61,12 @@ def main():
                     help="per-kernel durations from a rocprofv3 --kernel-trace database (no GPU use)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="HBM bytes per filter launch from two rocprofv3 --pmc runs of --profile-filter")
+    ap.add_argument("--workload", choices=("c3", "c2", "c4"), default="c3",
+                    help="c3: the headline (32^3x64, N_ev 220); c2: 24^3x48 N_ev 120, every timeslice through the "
+                         "plain drop-in and the batched solver; c4: 48^3x96 N_ev 200 in HBM-sized groups")
+    ap.add_argument("--stream-ceiling", action="store_true",
+                    help="only run the streaming-bandwidth probe (laph_eigsys_amd/stream_ceiling)")
+    ap.add_argument("--no-stream", action="store_true", help="skip the streaming-bandwidth probe")
     ap.add_argument("--last", type=int, default=0,
                     help="with --summarize-*: the isolated filter replays are the last LAST filter dispatches")
     args = ap.parse_args()
@@ -70,12 +76,21 @@ def main():
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc, args.last)
         return
+    if args.stream_ceiling:
+        print(json.dumps(stream_ceiling()))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); timeslices are "
               "independent, so N GPUs would each run their own Lt/N share with no exchange.", file=sys.stderr)
         sys.exit(2)
 
     from laph_eigsys_amd import lapheig as lp
+    if args.workload == "c2":
+        workload_c2(lp, args)
+        return
+    if args.workload == "c4":
+        workload_c4(lp, args)
+        return
     if args.profile_filter:
         profile_filter(lp, args)
         return
@@ -122,6 +137,13 @@ def main():
                 "kernel": f"k_filter_step32{'x2' if st.filter_block == 128 else ''} (block {st.filter_block})",
                 "avg_launch_us": ms_launch * 1e3, "bytes_per_launch": bytes_launch,
                 "in_solve_GBps": tot["fbytes"] / tot["filter"] / 1e9 if tot["filter"] > 0 else None}
+
+    if not args.no_stream:
+        # what a plain streaming kernel reaches for the filter's access mix
+        # (3 reads + 1 write, non-temporal) on this device: the practical ceiling
+        sc = stream_ceiling()
+        roofline["stream_3r1w_GBps"] = sc.get("3r1w_nt_GBps")
+        roofline["frac_of_stream"] = achieved / 1e9 / sc["3r1w_nt_GBps"] if sc.get("3r1w_nt_GBps") else None
 
     # plain FP64 drop-in on a few timeslices of the same field
     lamC = 1.35 * abs(lam_last)
@@ -244,6 +266,97 @@ def summarize_pmc(fetch_csv, write_csv, last):
     print(json.dumps({"filter_dispatches": [nf, nw], "fetch_KiB": f, "write_KiB": w,
                       "copy_fetch_KiB": fc, "copy_write_KiB": wc, "fetch_scale": kf, "write_scale": kw,
                       "traffic_bytes_per_launch": traffic}))
+
+
+def workload_c2(lp, args):
+    """24^3 x 48, eps 0.4, HYP 0.62/0.62/3, N_ev 120, DEG 12, tol 1e-10: every
+    timeslice through the plain FP64 drop-in (the MatShell path) and through
+    the batched solver; eigenvalues compared timeslice by timeslice."""
+    L2, T2, NEV2, DEG2 = 24, 48, 120, 12
+    cfg = lp.synth_config(1227, 0.4, L2, T2)
+    vol = 72 * L2 ** 3
+    sync = device_sync()
+    b = lp.Batch(L2, DEG2, LAMBDA_L, 0.0, T2, NEV2, tol=TOL, seed=SEED)
+    for t in range(T2):
+        b.set_timeslice(t, cfg[t * vol:(t + 1) * vol], 0.62, 0.62, 3)
+    b.solve()  # warm-up (layout autotuning, graph capture)
+    sync()
+    t0 = time.perf_counter()
+    st = b.solve()
+    sync()
+    t_batch = time.perf_counter() - t0
+    evb = [b.get(t, want_vectors=False)[0] for t in range(T2)]
+    b.close()
+    lam_c = 1.35 * max(-e[-1] for e in evb)
+    t_plain, dev, restarts = 0.0, 0.0, []
+    for t in range(T2):
+        with lp.Tslice(L2, DEG2, LAMBDA_L, lam_c, ts_config=cfg[t * vol:(t + 1) * vol]) as sl:
+            sl.smearing_hyp(0.62, 0.62, 3)
+            sync()
+            tp = time.perf_counter()
+            theta, _, pst = sl.eps_solve(NEV2, tol=TOL, want_vectors=False)
+            t_plain += time.perf_counter() - tp
+        restarts.append(pst.restarts)
+        ev = lp.recover_spectrum(theta, DEG2, LAMBDA_L, lam_c)
+        dev = max(dev, float(np.max(np.abs(ev - evb[t]) / np.abs(evb[t]))))
+    print(json.dumps({"workload": "24^3x48 eps=0.4 N_ev=120 DEG 12 tol 1e-10", "timeslices": T2,
+                      "batched_ts_per_s": T2 / t_batch, "batched_cycles": st.iterations,
+                      "plain_fp64_ts_per_s": T2 / t_plain, "plain_lambda_c": lam_c,
+                      "plain_restarts_max": max(restarts), "max_rel_eigenvalue_difference": dev}))
+
+
+def workload_c4(lp, args):
+    """48^3 x 96, eps 0.3, HYP 0.62/0.62/3, N_ev 200, DEG 16, tol 1e-10: the
+    96 timeslices stream through groups whose work buffers fit HBM (the
+    per-configuration driver's plan); eigenvectors stay on the device."""
+    from laph_eigsys_amd import ev_ts
+    L4, T4, NEV4 = 48, 96, 200
+    groups = ev_ts.plan_groups(T4, L4, NEV4, 200.0)
+    cfg = lp.synth_config(SEED, EPS, L4, T4)
+    vol = 72 * L4 ** 3
+    sync = device_sync()
+    gn = groups[0][1]
+    b = lp.Batch(L4, DEG, LAMBDA_L, 0.0, gn, NEV4, tol=TOL, seed=SEED)
+    for k in range(gn):
+        b.set_timeslice(k, cfg[k * vol:(k + 1) * vol], 0.62, 0.62, 3)
+    b.solve()  # warm-up on the first group (layout autotuning, graph capture)
+    sync()
+    t_solve, t_set, cycles, maxres = 0.0, 0.0, 0, 0.0
+    for g0, n in groups:
+        assert n == gn, "groups of equal size"
+        ts0 = time.perf_counter()
+        for k in range(n):
+            b.set_timeslice(k, cfg[(g0 + k) * vol:(g0 + k + 1) * vol], 0.62, 0.62, 3)
+        sync()
+        t_set += time.perf_counter() - ts0
+        t0 = time.perf_counter()
+        st = b.solve()
+        sync()
+        t_solve += time.perf_counter() - t0
+        cycles, maxres = max(cycles, st.iterations), max(maxres, st.max_residual)
+        assert st.converged == n
+        print(f"[bench] c4 group {g0}..{g0 + n - 1}: {st.seconds:.2f}s cycles={st.iterations}", file=sys.stderr,
+              flush=True)
+    ms, by = b.time_filter(reps=5)
+    b.close()
+    print(json.dumps({"workload": "48^3x96 eps=0.3 N_ev=200 DEG 16 tol 1e-10", "timeslices": T4,
+                      "groups": [n for _, n in groups], "ts_per_s_solve": T4 / t_solve,
+                      "ts_per_s_with_upload_hyp": T4 / (t_solve + t_set), "cycles_max": cycles,
+                      "max_residual": maxres, "filter_block": st.filter_block, "filter_strip": st.filter_strip,
+                      "filter_GBps": by / ms / 1e6, "filter_us_per_launch": ms * 1e3}))
+
+
+def stream_ceiling():
+    """JSON of laph_eigsys_amd/stream_ceiling (built with the library): best
+    GB/s of float4 streaming kernels for 1-3 reads + 1 write and read-only."""
+    import subprocess
+    exe = os.path.join(ROOT, "laph_eigsys_amd", "stream_ceiling")
+    if not os.path.exists(exe):
+        raise RuntimeError(f"{exe} missing: run __graft_entry__.build()")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        raise RuntimeError(f"stream_ceiling failed ({r.returncode}): {r.stderr.strip()}")
+    return json.loads(r.stdout)
 
 
 def device_sync():
