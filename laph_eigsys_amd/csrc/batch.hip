@@ -337,6 +337,10 @@ static int orthonormalise_and_rr(const View& v, Act q, bool explicit_qr) {
     if (explicit_qr || pass == 1) {
       LAPH_TRY(cholqr(v, q));
       LAPH_TRY(cholqr(v, q));
+    } else {
+      // unit columns: equilibrates the Gram whose Cholesky the generalized
+      // Rayleigh-Ritz takes (the span is unchanged)
+      LAPH_HIP(launch_normalize_columns(v.X(), b->cn + v.nk(), b->n, b->K, v.ln->nt, q.l, v.ln->st));
     }
     LAPH_HIP(launch_lap_block64(b->U64 + v.lo(), v.X(), v.AX(), b->p.L, b->K, v.ln->nt, b->lock_blk + v.ln->t0,
                                 b->strip, v.ln->st));
@@ -628,7 +632,7 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
   }
   // start block (network proposal or seeded Gaussian), CholQR2, first Rayleigh-Ritz
   if (!warm) LAPH_HIP(launch_random_block(v.X(), (size_t)nt * b->per_ts, b->o.seed + 0x1000193ull * ln->t0, ln->st));
-  LAPH_TRY(orthonormalise_and_rr(v, Act{0, K}, true));
+  LAPH_TRY(orthonormalise_and_rr(v, Act{0, K}, b->qr_cycles > 0));
   std::vector<double> lc(nt);
   int active = 0;
   LAPH_TRY(residuals_and_window(v, lc, &active));
@@ -678,7 +682,7 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
     LAPH_TRY(project_locked(v, q));
     // explicit CholQR2 while the block is far from orthonormal (first
     // cycles from a random start), Cholesky inside Rayleigh-Ritz after
-    LAPH_TRY(orthonormalise_and_rr(v, q, it < 2));
+    LAPH_TRY(orthonormalise_and_rr(v, q, it < b->qr_cycles));
     LAPH_HIP(hipEventRecord(ln->ev[2], ln->st));
     LAPH_TRY(residuals_and_window(v, lc, &active));
     if (b->trace && ln->t0 == 0) {  // per-cycle trace of the first timeslice
@@ -860,6 +864,7 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     b->trace = std::getenv("LAPH_TRACE") != nullptr;
     if (const char* e = std::getenv("LAPH_WARM")) b->warm_cycles = std::max(0, std::atoi(e));
     b->force_shift = std::getenv("LAPH_SHIFTED_CHOLQR") != nullptr;
+    if (const char* e = std::getenv("LAPH_QR_CYCLES")) b->qr_cycles = std::max(0, std::atoi(e));
     b->strip = strip > 0 ? strip : p->L;
     b->strip_fixed = strip > 0;
     b->bf16 = o->filter_storage == 1;

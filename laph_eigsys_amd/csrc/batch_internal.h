@@ -91,6 +91,7 @@ struct laph_batch {
   int strip = 0;  // sweep strip width of the stencil kernels (divides L; L = x-major planes)
   bool strip_fixed = false;  // LAPH_STRIP given
   bool tuned = false;        // filter layout chosen (first solve)
+  int qr_cycles = 0;  // cycles (and the start block) orthonormalised by explicit CholQR2 (LAPH_QR_CYCLES)
   bool force_shift = false;  // every CholQR pass starts with a shifted one (LAPH_SHIFTED_CHOLQR)
 };
 
