@@ -338,8 +338,22 @@ def workload_c4(lp, args):
         print(f"[bench] c4 group {g0}..{g0 + n - 1}: {st.seconds:.2f}s cycles={st.iterations}", file=sys.stderr,
               flush=True)
     ms, by = b.time_filter(reps=5)
+    # streaming device -> file of two timeslices' eigensystems (1.06 GB of
+    # eigenvectors each) into a scratch directory that is removed afterwards
+    import shutil
+    import tempfile
+    tmp = tempfile.mkdtemp(prefix="laph_c4_", dir=os.environ.get("TMPDIR", "/tmp"))
+    try:
+        tw = time.perf_counter()
+        for k in range(2):
+            b.write(k, tmp, 1, k)
+        t_write = time.perf_counter() - tw
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    wbytes = 2 * 3 * L4 ** 3 * NEV4 * 16
     b.close()
     print(json.dumps({"workload": "48^3x96 eps=0.3 N_ev=200 DEG 16 tol 1e-10", "timeslices": T4,
+                      "write_GBps": wbytes / t_write / 1e9, "write_seconds_per_timeslice": t_write / 2,
                       "groups": [n for _, n in groups], "ts_per_s_solve": T4 / t_solve,
                       "ts_per_s_with_upload_hyp": T4 / (t_solve + t_set), "cycles_max": cycles,
                       "max_residual": maxres, "filter_block": st.filter_block, "filter_strip": st.filter_strip,

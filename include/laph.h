@@ -134,6 +134,21 @@ int laph_write_eig_sys_bin(const char* dir, const char* prefix, int config, int 
 /* write_eigenvalues_bin (read_write.cpp:237-250). */
 int laph_write_eigenvalues_bin(const char* dir, const char* prefix, int config, int t,
                                int nev, const double* values);
+/* read_evectors_bin_ts (read_write.cpp:40-96): nev eigenvectors of n complex
+ * doubles from "<dir>/<prefix>.%04d.%03d" into V (nev x n, row i =
+ * eigenvector i, i.e. the column-major file order), then check_trace.
+ * LAPH_ERR_IO: no such file; LAPH_ERR_FORMAT: fewer than nev vectors in it
+ * (the reference exits in eof_check); LAPH_ERR_TRACE: the guard fails (the
+ * reference exits with "Eigenvectors damaged"); V holds what was read. */
+int laph_read_evectors_bin_ts(const char* dir, const char* prefix, int config, int t, int n, int nev,
+                              double* V);
+/* read_eigenvalues_bin (read_write.cpp:137-152): nev doubles.  A missing file
+ * is LAPH_ERR_IO (the reference silently keeps its old vector), a short one
+ * LAPH_ERR_FORMAT. */
+int laph_read_eigenvalues_bin(const char* dir, const char* prefix, int config, int t, int nev, double* values);
+/* read_eigenvalues_ascii (read_write.cpp:98-115): nev whitespace-separated
+ * values from "<prefix>.%03d.%04d" (timeslice before config, as there). */
+int laph_read_eigenvalues_ascii(const char* prefix, int config, int t, int nev, double* values);
 
 /* ------------------------------------------------------------------------
  * Gauge-field ingest (io.cpp:22-146) and the seeded synthetic field. */
@@ -199,6 +214,15 @@ int laph_batch_set_timeslice(laph_batch* b, int ts, const double* ts_config, dou
 int laph_batch_set_links(laph_batch* b, int ts, const double* links);
 /* Solve every timeslice of the batch. */
 int laph_batch_solve(laph_batch* b, laph_batch_stats* stats);
+/* The three files of one solved timeslice straight from device memory
+ * (ev_ts.cpp:304-318 for that timeslice): "<dir>/eigenvectors.%04d.%03d"
+ * (phase-fixed, column-major n x nev complex128, guarded by check_trace with
+ * V^H V computed on the device), "eigenvalues" and "phases".  The
+ * eigenvectors stream to the file through pinned chunks, copy of the next
+ * overlapping the write of the current.  trace_sum (4 doubles, may be null)
+ * receives tr(V^H V) and sum(V^H V) as (re, im, re, im). */
+int laph_batch_write(laph_batch* b, int ts, const char* dir, int config, int t, double* trace_sum);
+
 /* Device time of the FP32 filter-step kernel, alone on the device: replays
  * the captured filter of the first half of the timeslices (every column
  * active) reps times, timed with HIP events on its stream.  Outputs the

@@ -142,6 +142,13 @@ static void batch_free(laph_batch* b) {
     if (ln.rb) rocblas_destroy_handle(ln.rb);
     if (ln.st) (void)hipStreamDestroy(ln.st);
   }
+  for (auto e : b->wev)
+    if (e) (void)hipEventDestroy(e);
+  for (void* h : b->pin)
+    if (h) (void)hipHostFree(h);
+  if (b->wrb) rocblas_destroy_handle(b->wrb);
+  for (void* q : {(void*)b->wV, (void*)b->wph, (void*)b->wG})
+    if (q) (void)hipFree(q);
   void* bufs[] = {b->U64, b->U32, b->dec,  b->Unew, b->B[0], b->B[1],   b->B[2], b->R32,     b->P[0],
                   b->P[1], b->P[2], b->G,  b->G2,   b->P2,   b->mu,     b->E,    b->info,    b->nrm2,
                   b->lam,  b->cts,  b->gam, b->coef, b->cab, b->first_blk, b->lock_blk, b->cn, b->coef2, b->jsweeps};
@@ -989,6 +996,86 @@ int laph_batch_get(laph_batch* b, int ts, double* evals, double* V_fix, double* 
     LAPH_HIP(e);
   }
   return LAPH_OK;
+}
+
+int laph_batch_write(laph_batch* b, int ts, const char* dir, int config, int t, double* trace_sum) {
+  LAPH_CHECK(b && ts >= 0 && ts < b->nts && dir, LAPH_ERR_INVALID, "laph_batch_write: bad arguments");
+  LAPH_CHECK(b->solved, LAPH_ERR_INVALID, "laph_batch_write: batch not solved");
+  LAPH_HIP(hipSetDevice(b->p.device));
+  return laph::guarded([&]() -> int {
+    const int n = b->n, nev = b->nev, K = b->K;
+    const size_t CH = (size_t)64 << 20;  // staging chunk
+    if (!b->wV) {
+      LAPH_HIP(hipMalloc(&b->wV, (size_t)n * nev * sizeof(double2)));
+      LAPH_HIP(hipMalloc(&b->wph, nev * sizeof(double)));
+      LAPH_HIP(hipMalloc(&b->wG, (size_t)nev * nev * sizeof(double2)));
+      LAPH_CHECK(rocblas_create_handle(&b->wrb) == rocblas_status_success &&
+                     rocblas_set_stream(b->wrb, b->st) == rocblas_status_success,
+                 LAPH_ERR_HIP, "laph_batch_write: rocBLAS handle");
+      for (int i = 0; i < 2; ++i) {
+        LAPH_HIP(hipHostMalloc(&b->pin[i], CH, hipHostMallocDefault));
+        LAPH_HIP(hipEventCreateWithFlags(&b->wev[i], hipEventDisableTiming));
+      }
+    }
+    // phase-fixed eigenvectors (eigensystem.cpp:4-30) and their Gram V^H V
+    LAPH_HIP(laph::launch_extract_phase(b->B[0] + (size_t)ts * b->per_ts, b->wV, b->wph, n, K, nev, b->st));
+    const laph::cplx one(1.0, 0.0), zero(0.0, 0.0);
+    LAPH_RB(rocblas_zgemm(b->wrb, rocblas_operation_conjugate_transpose, rocblas_operation_none, nev, nev, n, &one,
+                          laph::zc(b->wV), n, laph::zc(b->wV), n, &zero, laph::zc(b->wG), nev));
+    std::vector<double2> G((size_t)nev * nev);
+    std::vector<double> ph(nev), ev(nev);
+    LAPH_HIP(hipMemcpyAsync(G.data(), b->wG, G.size() * sizeof(double2), hipMemcpyDeviceToHost, b->st));
+    LAPH_HIP(hipMemcpyAsync(ph.data(), b->wph, nev * sizeof(double), hipMemcpyDeviceToHost, b->st));
+    LAPH_HIP(hipStreamSynchronize(b->st));
+    double tr[2] = {0, 0}, sm[2] = {0, 0};
+    for (int i = 0; i < nev; ++i) {
+      tr[0] += G[(size_t)i * nev + i].x;
+      tr[1] += G[(size_t)i * nev + i].y;
+      for (int j = 0; j < nev; ++j) {
+        sm[0] += G[(size_t)i * nev + j].x;
+        sm[1] += G[(size_t)i * nev + j].y;
+      }
+    }
+    if (trace_sum) {
+      trace_sum[0] = tr[0];
+      trace_sum[1] = tr[1];
+      trace_sum[2] = sm[0];
+      trace_sum[3] = sm[1];
+    }
+    const double eps = 10e-10;  // read_write.cpp:24-31 (one-sided, as there)
+    LAPH_CHECK(!(std::fabs(tr[0]) - nev > eps || std::fabs(tr[1]) > eps || std::fabs(sm[0]) - nev > eps ||
+                 std::fabs(sm[1]) > eps),
+               LAPH_ERR_TRACE, "timeslice %d: eigenvectors damaged, tr(V^H V) = (%.12g, %.3g), sum = (%.12g, %.3g)", t,
+               tr[0], tr[1], sm[0], sm[1]);
+    // eigenvectors: chunk i+1 copies while chunk i is written
+    char file[4096];
+    std::snprintf(file, sizeof file, "%s/eigenvectors.%04d.%03d", dir, config, t);
+    FILE* f = std::fopen(file, "wb");
+    LAPH_CHECK(f, LAPH_ERR_IO, "could not create %s", file);
+    const size_t total = (size_t)n * nev * sizeof(double2);
+    const char* src = reinterpret_cast<const char*>(b->wV);
+    const size_t nch = (total + CH - 1) / CH;
+    hipError_t e = hipMemcpyAsync(b->pin[0], src, std::min(CH, total), hipMemcpyDeviceToHost, b->st);
+    if (e == hipSuccess) e = hipEventRecord(b->wev[0], b->st);
+    bool ok = e == hipSuccess;
+    for (size_t i = 0; ok && i < nch; ++i) {
+      const size_t len = std::min(CH, total - i * CH);
+      e = hipEventSynchronize(b->wev[i % 2]);
+      if (e == hipSuccess && i + 1 < nch) {
+        e = hipMemcpyAsync(b->pin[(i + 1) % 2], src + (i + 1) * CH, std::min(CH, total - (i + 1) * CH),
+                           hipMemcpyDeviceToHost, b->st);
+        if (e == hipSuccess) e = hipEventRecord(b->wev[(i + 1) % 2], b->st);
+      }
+      ok = e == hipSuccess && std::fwrite(b->pin[i % 2], 1, len, f) == len;
+    }
+    const bool closed = std::fclose(f) == 0;
+    (void)hipStreamSynchronize(b->st);  // no copy left in flight into the staging chunks
+    LAPH_HIP(e);
+    LAPH_CHECK(ok && closed, LAPH_ERR_IO, "write incomplete: %s", file);
+    for (int j = 0; j < nev; ++j) ev[j] = -b->h_mu[(size_t)ts * K + j];
+    LAPH_TRY(laph_write_eigenvalues_bin(dir, "eigenvalues", config, t, nev, ev.data()));
+    return laph_write_eigenvalues_bin(dir, "phases", config, t, nev, ph.data());
+  });
 }
 
 }  // extern "C"

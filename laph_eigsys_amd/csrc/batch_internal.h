@@ -92,6 +92,14 @@ struct laph_batch {
   bool strip_fixed = false;  // LAPH_STRIP given
   bool tuned = false;        // filter layout chosen (first solve)
   int qr_cycles = 0;  // cycles (and the start block) orthonormalised by explicit CholQR2 (LAPH_QR_CYCLES)
+  // laph_batch_write: phase-fixed copy, phases, V^H V, rocBLAS handle on st,
+  // two pinned staging chunks and their events (created on first use)
+  double2* wV = nullptr;
+  double* wph = nullptr;
+  double2* wG = nullptr;
+  rocblas_handle wrb = nullptr;
+  void* pin[2] = {nullptr, nullptr};
+  hipEvent_t wev[2] = {nullptr, nullptr};
   bool force_shift = false;  // every CholQR pass starts with a shifted one (LAPH_SHIFTED_CHOLQR)
 };
 

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <thread>
@@ -396,6 +397,46 @@ int laph_write_eigenvalues_bin(const char* dir, const char* prefix, int config, 
   LAPH_CHECK(out, LAPH_ERR_IO, "could not create %s", file);
   out.write(reinterpret_cast<const char*>(values), (std::streamsize)nev * sizeof(double));
   LAPH_CHECK(out.good(), LAPH_ERR_IO, "write incomplete: %s", file);
+  return LAPH_OK;
+}
+
+int laph_read_evectors_bin_ts(const char* dir, const char* prefix, int config, int t, int n, int nev,
+                              double* V) {
+  LAPH_CHECK(dir && prefix && V && n > 0 && nev > 0, LAPH_ERR_INVALID, "laph_read_evectors_bin_ts: bad arguments");
+  char file[4096];
+  std::snprintf(file, sizeof file, "%s/%s.%04d.%03d", dir, prefix, config, t);
+  std::ifstream in(file, std::ifstream::binary);
+  LAPH_CHECK(in, LAPH_ERR_IO, "could not open %s", file);
+  for (int i = 0; i < nev; ++i) {  // one eigenvector after the other, as read_write.cpp:58-62
+    in.read(reinterpret_cast<char*>(V + (size_t)2 * n * i), (std::streamsize)2 * n * sizeof(double));
+    LAPH_CHECK(in.gcount() == (std::streamsize)(2 * n * sizeof(double)), LAPH_ERR_FORMAT,
+               "%s: %d of %d eigenvectors of length %d in the file", file, i, nev, n);
+  }
+  return laph_check_trace(n, nev, V, nullptr, nullptr);  // read_write.cpp:63-66
+}
+
+int laph_read_eigenvalues_bin(const char* dir, const char* prefix, int config, int t, int nev, double* values) {
+  LAPH_CHECK(dir && prefix && values && nev >= 0, LAPH_ERR_INVALID, "laph_read_eigenvalues_bin: bad arguments");
+  char file[4096];
+  std::snprintf(file, sizeof file, "%s/%s.%04d.%03d", dir, prefix, config, t);
+  std::ifstream in(file, std::ifstream::binary);
+  LAPH_CHECK(in, LAPH_ERR_IO, "could not open %s", file);
+  in.read(reinterpret_cast<char*>(values), (std::streamsize)nev * sizeof(double));
+  LAPH_CHECK(in.gcount() == (std::streamsize)(nev * sizeof(double)), LAPH_ERR_FORMAT, "%s: fewer than %d values",
+             file, nev);
+  return LAPH_OK;
+}
+
+int laph_read_eigenvalues_ascii(const char* prefix, int config, int t, int nev, double* values) {
+  LAPH_CHECK(prefix && values && nev >= 0, LAPH_ERR_INVALID, "laph_read_eigenvalues_ascii: bad arguments");
+  char file[4096];
+  std::snprintf(file, sizeof file, "%s.%03d.%04d", prefix, t, config);  // read_write.cpp:104
+  std::ifstream in(file);
+  LAPH_CHECK(in, LAPH_ERR_IO, "could not open %s", file);
+  for (int i = 0; i < nev; ++i) {
+    in >> values[i];
+    LAPH_CHECK(!in.fail(), LAPH_ERR_FORMAT, "%s: value %d of %d unreadable", file, i, nev);
+  }
   return LAPH_OK;
 }
 

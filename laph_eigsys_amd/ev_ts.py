@@ -10,7 +10,8 @@ covariant Laplacian of each timeslice nearest zero and writes, per timeslice t,
 <result_path>/eigenvectors.%04d.%03d (column-major complex128, checked with
 tr(V^H V) before writing), eigenvalues.%04d.%03d and phases.%04d.%03d
 (read_write.cpp:176-201, 237-250), after fixing the phases
-(eigensystem.cpp:4-30).
+(eigensystem.cpp:4-30) or, with --phase_reset, resetting them to a stored
+convention (eigensystem.cpp:32-57).
 
 Solvers:
   plain    the drop-in, ev_ts.cpp:176-303 step for step: per timeslice a
@@ -93,15 +94,6 @@ class IO:
         print(f"Entries in Matrix: {g('MAT_ENTRIES')}", file=out)
 
 
-def read_eigenvalues_bin(path, prefix, config, t, nev):
-    """read_write.cpp:137-152: nev doubles from <path>/<prefix>.%04d.%03d;
-    None when the file is absent (the reference then keeps its old vector)."""
-    fn = f"{path}/{prefix}.{config:04d}.{t:03d}"
-    if not os.path.exists(fn):
-        return None
-    return np.fromfile(fn, dtype=np.float64, count=nev)
-
-
 def batch_bytes_per_timeslice(L, nev, nsub=0):
     """Device bytes of one timeslice in a batch: three FP64 blocks, four FP32
     filter buffers, three K x K FP64 matrices, FP64 + FP32 links."""
@@ -127,24 +119,39 @@ def plan_groups(todo, L, nev, mem_gb, group=0, nsub=0):
     return out
 
 
-def finish_timeslice(pars, config, t, ev, V, phase_reset, log):
-    """ev_ts.cpp:296-325 for one timeslice: (phase reset), phase fix, files,
-    trace prints.  V: (nev, n) with row i = eigenvector i."""
+def finish_timeslice(pars, config, t, ev, V, phase_reset, log, batch=None, k=None):
+    """ev_ts.cpp:296-325 for one timeslice: phase fix (or reset), the three
+    files, the trace prints.  V: (nev, n) with row i = eigenvector i, or None
+    with `batch`/`k`: the solved batch writes its timeslice k straight from
+    device memory (phase fix and the V^H V of the guard on the device).
+
+    With phase_reset the files hold the eigenvectors in the old phase
+    convention, arg V(0, n) = old phase n, and the phases file the old
+    phases.  In the reference the fix_phase call that follows reset_phase
+    (ev_ts.cpp:302-309) overwrites the reset, so its files are phase-fixed
+    either way; that defect is not reproduced."""
     nev = pars.get_int("NEV")
     res = pars.get_path("res")
     if phase_reset:
-        # ev_ts.cpp:296-302 reads and prints the old phases; its fix_phase
-        # call right after (302-303) overwrites the reset vectors, so the
-        # files hold the phase-fixed system either way
-        old = read_eigenvalues_bin(pars.get_path("phase_path"), "phases", config, t, nev)
-        log(f"Resetting old phases: {None if old is None else old.tolist()}")
+        old = lp.read_eigenvalues_bin(pars.get_path("phase_path"), "phases", config, t, nev)
+        if V is None:
+            ev, V, _, _ = batch.get(k)
+        Vr = lp.reset_phase(V, old)
+        lp.write_eig_sys_bin(res, "eigenvectors", config, t, Vr)
+        lp.write_eigenvalues_bin(res, "eigenvalues", config, t, ev)
+        lp.write_eigenvalues_bin(res, "phases", config, t, old)
+        log(f"{t}: phases reset to {pars.get_path('phase_path')}/phases.{config:04d}.{t:03d}")
+        return
+    if V is None:
+        trc, sm = batch.write(k, res, config, t)
+        log(f"{t}: tr(V.adj() * V) = {trc}  sum(V.adj() * V) = {sm}")
+        return
     Vf, phase = lp.fix_phase(V)
     lp.write_eig_sys_bin(res, "eigenvectors", config, t, Vf)
     lp.write_eigenvalues_bin(res, "eigenvalues", config, t, ev)
     lp.write_eigenvalues_bin(res, "phases", config, t, phase)
     ok, trc, sm = lp.check_trace(V)
     log(f"{t}: tr(V.adj() * V) = {trc}  sum(V.adj() * V) = {sm}  sum(V) = {complex(V.sum())}")
-    return phase
 
 
 def run(pars, config, solver="batched", tstart=0, tend=None, group=0, mem_gb=200.0, nsub=0, tol=1e-10,
@@ -205,8 +212,7 @@ def run(pars, config, solver="batched", tstart=0, tend=None, group=0, mem_gb=200
                 f"max residual {st.max_residual:.2e}")
             tw = time.perf_counter()
             for k in range(gn):
-                ev, V, _, _ = b.get(k)
-                finish_timeslice(pars, config, tstart + g0 + k, ev, V, phase_reset, log)
+                finish_timeslice(pars, config, tstart + g0 + k, None, None, phase_reset, log, batch=b, k=k)
             out["write_seconds"] += time.perf_counter() - tw
     finally:
         if b is not None:
@@ -217,7 +223,8 @@ def run(pars, config, solver="batched", tstart=0, tend=None, group=0, mem_gb=200
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--conf", type=int, required=True, help="trajectory number of the gauge configuration")
-    ap.add_argument("--phase_reset", type=int, default=0, help="read old phases from phase_path")
+    ap.add_argument("--phase_reset", type=int, default=0,
+                    help="write the eigenvectors with the phases of <phase_path>/phases.%%04d.%%03d")
     ap.add_argument("--params", default="parameters.txt")
     ap.add_argument("--solver", choices=("batched", "plain"), default="batched")
     ap.add_argument("--tstart", type=int, default=0)

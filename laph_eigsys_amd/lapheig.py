@@ -91,6 +91,9 @@ SIGNATURES = {
     "laph_check_trace": ([_I, _I, _dp, _dp, _dp], _I),
     "laph_write_eig_sys_bin": ([_S, _S, _I, _I, _I, _I, _dp], _I),
     "laph_write_eigenvalues_bin": ([_S, _S, _I, _I, _I, _dp], _I),
+    "laph_read_evectors_bin_ts": ([_S, _S, _I, _I, _I, _I, _dp], _I),
+    "laph_read_eigenvalues_bin": ([_S, _S, _I, _I, _I, _dp], _I),
+    "laph_read_eigenvalues_ascii": ([_S, _I, _I, _I, _dp], _I),
     "laph_read_lime": ([_S, _I, _I, _I, _I, _dp], _I),
     "laph_write_lime": ([_S, _I, _I, _dp], _I),
     "laph_synth_config": ([ctypes.c_ulonglong, _D, _I, _I, _dp], _I),
@@ -101,6 +104,7 @@ SIGNATURES = {
     "laph_batch_solve": ([_VP, ctypes.POINTER(laph_batch_stats)], _I),
     "laph_batch_get": ([_VP, _I, _dp, _dp, _dp, _dp], _I),
     "laph_batch_time_filter": ([_VP, _I, _dp, _dp], _I),
+    "laph_batch_write": ([_VP, _I, _S, _I, _I, _dp], _I),
     "laph_net_create": ([ctypes.POINTER(_VP), _I, ctypes.c_ulonglong, _I], _I),
     "laph_net_destroy": ([_VP], _I),
     "laph_net_num_params": ([_VP], _I),
@@ -221,6 +225,31 @@ def write_eigenvalues_bin(result_path, prefix, config, t, values):
     values = np.ascontiguousarray(values, dtype=np.float64)
     _check(lib().laph_write_eigenvalues_bin(os.fsencode(result_path), prefix.encode(), config, t,
                                             len(values), _p(values)), "laph_write_eigenvalues_bin")
+
+
+def read_evectors_bin_ts(result_path, prefix, config, t, n, nev):
+    """read_write.cpp:40-96 -> V (nev, n), row i = eigenvector i; LaphError on
+    a missing or short file and when check_trace fails."""
+    V = np.empty((nev, n), dtype=np.complex128)
+    _check(lib().laph_read_evectors_bin_ts(os.fsencode(result_path), prefix.encode(), config, t, n, nev, _p(V)),
+           "laph_read_evectors_bin_ts")
+    return V
+
+
+def read_eigenvalues_bin(result_path, prefix, config, t, nev):
+    """read_write.cpp:137-152."""
+    out = np.empty(nev)
+    _check(lib().laph_read_eigenvalues_bin(os.fsencode(result_path), prefix.encode(), config, t, nev, _p(out)),
+           "laph_read_eigenvalues_bin")
+    return out
+
+
+def read_eigenvalues_ascii(prefix, config, t, nev):
+    """read_write.cpp:98-115: "<prefix>.%03d.%04d" % (t, config)."""
+    out = np.empty(nev)
+    _check(lib().laph_read_eigenvalues_ascii(os.fsencode(prefix), config, t, nev, _p(out)),
+           "laph_read_eigenvalues_ascii")
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -345,6 +374,13 @@ class Batch:
         ms, by = ctypes.c_double(), ctypes.c_double()
         _check(lib().laph_batch_time_filter(self.h, reps, ctypes.byref(ms), ctypes.byref(by)), "laph_batch_time_filter")
         return ms.value, by.value
+
+    def write(self, ts, result_path, config, t):
+        """eigenvectors / eigenvalues / phases files of timeslice ts (file index t)
+        straight from the device -> (tr(V^H V), sum(V^H V))."""
+        ts_ = np.zeros(4)
+        _check(lib().laph_batch_write(self.h, ts, os.fsencode(result_path), config, t, _p(ts_)), "laph_batch_write")
+        return complex(ts_[0], ts_[1]), complex(ts_[2], ts_[3])
 
     def get(self, ts, want_vectors=True):
         ev = np.empty(self.nev)

@@ -48,3 +48,33 @@ def test_groups_cover_range_and_fit():
     for todo in range(1, 40):
         g = ev_ts.plan_groups(todo, 24, 120, 5.0)
         assert sum(n for _, n in g) == todo and all(a == sum(n for _, n in g[:i]) for i, (a, _) in enumerate(g))
+
+
+def test_set_phase_resets_to_stored_phases(tmp_path):
+    """set_phase (set_phase.cpp:13-58): stored eigenvectors come back with
+    arg V(0, n) = the stored old phase n, otherwise unchanged up to that
+    phase; the old phases are written beside them."""
+    import numpy as np
+    from laph_eigsys_amd import lapheig as lp
+    from laph_eigsys_amd import set_phase
+    L, nev, conf, t = 3, 5, 1300, 7
+    n = 3 * L ** 3
+    rng = np.random.default_rng(3)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, nev)) + 1j * rng.standard_normal((n, nev)))
+    V = Q.T.copy()
+    old = rng.uniform(-np.pi, np.pi, nev)
+    for d in ("in", "old", "out"):
+        (tmp_path / d).mkdir()
+    lp.write_eig_sys_bin(str(tmp_path / "in"), "eigenvectors", conf, t, V)
+    lp.write_eigenvalues_bin(str(tmp_path / "old"), "phases", conf, t, old)
+    f = tmp_path / "p.txt"
+    f.write_text(f"in_path = {tmp_path / 'in'}\nphase_path = {tmp_path / 'old'}\nresult_path = {tmp_path / 'out'}\n"
+                 f"LT = 8\nLX = {L}\nNEV = {nev}\n")
+    set_phase.main(["--conf", str(conf), "--t", str(t), "--params", str(f)])
+    Vr = lp.read_evectors_bin_ts(str(tmp_path / "out"), "eigenvectors_reset", conf, t, n, nev)
+    np.testing.assert_allclose(np.angle(Vr[:, 0]), old, atol=1e-12)
+    ratio = Vr / V
+    np.testing.assert_allclose(np.abs(ratio), 1.0, atol=1e-12)
+    np.testing.assert_allclose(ratio, ratio[:, :1] * np.ones((1, n)), atol=1e-12)
+    np.testing.assert_array_equal(lp.read_eigenvalues_bin(str(tmp_path / "out"), "eigenvectors_old_phase",
+                                                          conf, t, nev), old)

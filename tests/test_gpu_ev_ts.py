@@ -48,3 +48,65 @@ def test_driver_files(gpu, tmp_path):
             results[(solver, t)] = ev
     for t in range(LT):
         np.testing.assert_allclose(results[("plain", t)], results[("batched", t)], rtol=1e-10, atol=1e-12)
+
+
+def test_batch_write_matches_host_path(gpu, tmp_path):
+    """laph_batch_write (device phase fix, device V^H V guard, chunked
+    pinned streaming) writes the same bytes as laph_batch_get followed by the
+    host writers."""
+    from laph_eigsys_amd import lapheig as lp
+    L, nts, nev, conf = 6, 2, 16, 42
+    cfg = N.synth_config(5, 0.4, L, nts)
+    b = lp.Batch(L, 8, 12.0, 0.0, nts, nev, nsub=64, tol=1e-10, seed=1, device=gpu)
+    vol = 72 * L ** 3
+    for k in range(nts):
+        b.set_timeslice(k, cfg[k * vol:(k + 1) * vol], 0.62, 0.62, 3)
+    b.solve()
+    (tmp_path / "dev").mkdir()
+    (tmp_path / "host").mkdir()
+    for k in range(nts):
+        trc, sm = b.write(k, str(tmp_path / "dev"), conf, 10 + k)
+        assert abs(trc - nev) < 1e-10 and abs(sm - nev) < 1e-9
+        ev, V, ph, _ = b.get(k)
+        lp.write_eig_sys_bin(str(tmp_path / "host"), "eigenvectors", conf, 10 + k, V)
+        lp.write_eigenvalues_bin(str(tmp_path / "host"), "eigenvalues", conf, 10 + k, ev)
+        lp.write_eigenvalues_bin(str(tmp_path / "host"), "phases", conf, 10 + k, ph)
+        for name in ("eigenvectors", "eigenvalues", "phases"):
+            f = f"{name}.{conf:04d}.{10 + k:03d}"
+            assert (tmp_path / "dev" / f).read_bytes() == (tmp_path / "host" / f).read_bytes(), f
+    b.close()
+
+
+def test_driver_phase_reset(gpu, tmp_path):
+    """ev_ts --phase_reset: the files hold the eigenvectors with the stored
+    phases (arg V(0, n) = stored phase n) and the stored phases."""
+    from laph_eigsys_amd import lapheig as lp
+    L, LT, nev, conf = 6, 2, 12, 7
+    cfg = N.synth_config(11, 0.4, L, LT)
+    (tmp_path / "confs").mkdir()
+    lp.write_lime(str(tmp_path / "confs" / f"conf.{conf:04d}"), cfg, LT, L)
+    (tmp_path / "old").mkdir()
+    rng = np.random.default_rng(0)
+    old = {t: rng.uniform(-np.pi, np.pi, nev) for t in range(LT)}
+    for t in range(LT):
+        lp.write_eigenvalues_bin(str(tmp_path / "old"), "phases", conf, t, old[t])
+    runs = {}
+    for mode in (0, 1):
+        out = tmp_path / f"out{mode}"
+        (tmp_path / f"p{mode}.txt").write_text(
+            f"config_path = {tmp_path / 'confs'}\nresult_path = {out}\nphase_path = {tmp_path / 'old'}\n"
+            f"LT = {LT}\nLX = {L}\nNEV = {nev}\nalpha_1 = 0.62\nalpha_2 = 0.62\niter = 3\nLAM_L = 12.0\n"
+            f"LAM_C = 1.0\nDEG = 8\n")
+        pars = ev_ts.IO().set_values(str(tmp_path / f"p{mode}.txt"))
+        ev_ts.run(pars, conf, solver="batched", phase_reset=bool(mode), device=gpu, log=lambda m: None)
+        runs[mode] = out
+    n = 3 * L ** 3
+    for t in range(LT):
+        Vf = lp.read_evectors_bin_ts(str(runs[0]), "eigenvectors", conf, t, n, nev)
+        Vr = lp.read_evectors_bin_ts(str(runs[1]), "eigenvectors", conf, t, n, nev)
+        np.testing.assert_allclose(np.angle(Vr[:, 0]), old[t], atol=1e-10)
+        np.testing.assert_array_equal(lp.read_eigenvalues_bin(str(runs[1]), "phases", conf, t, nev), old[t])
+        ev = lp.read_eigenvalues_bin(str(runs[1]), "eigenvalues", conf, t, nev)
+        iso = [i for i in range(1, nev - 1) if min(ev[i - 1] - ev[i], ev[i] - ev[i + 1]) > 1e-4]
+        for i in iso:  # the same eigenvector, phase-fixed once and reset once
+            np.testing.assert_allclose(Vr[i] * np.exp(-1j * old[t][i]), Vf[i], atol=1e-8)

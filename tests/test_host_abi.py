@@ -117,3 +117,39 @@ def test_writers_byte_format(L, tmp_path):
         L.write_eig_sys_bin(str(tmp_path), "bad", 1, 1, 2 * V)  # read_write.cpp:183-186 guard
     assert e.value.code == -7
     assert not os.path.exists(tmp_path / "bad.0001.001")
+
+
+def test_readers_round_trip_and_errors(L, tmp_path):
+    """read_evectors_bin_ts / read_eigenvalues_bin / read_eigenvalues_ascii
+    (read_write.cpp:40-152): what the writers wrote comes back bit-exact; a
+    missing file, a short file and a damaged basis are errors (the reference
+    exits on the last two)."""
+    rng = np.random.default_rng(9)
+    n, nev = 30, 4
+    Q, _ = np.linalg.qr(rng.standard_normal((n, nev)) + 1j * rng.standard_normal((n, nev)))
+    V = Q.T.copy()
+    d = str(tmp_path)
+    L.write_eig_sys_bin(d, "eigenvectors", 1300, 5, V)
+    np.testing.assert_array_equal(L.read_evectors_bin_ts(d, "eigenvectors", 1300, 5, n, nev), V)
+    ev = -np.arange(1, nev + 1) / 7.0
+    L.write_eigenvalues_bin(d, "eigenvalues", 1300, 5, ev)
+    np.testing.assert_array_equal(L.read_eigenvalues_bin(d, "eigenvalues", 1300, 5, nev), ev)
+    with pytest.raises(L.LaphError) as e:
+        L.read_eigenvalues_bin(d, "eigenvalues", 1300, 6, nev)
+    assert e.value.code == -4
+    with pytest.raises(L.LaphError) as e:
+        L.read_eigenvalues_bin(d, "eigenvalues", 1300, 5, nev + 1)
+    assert e.value.code == -5
+    with pytest.raises(L.LaphError) as e:
+        L.read_evectors_bin_ts(d, "eigenvectors", 1300, 5, n, nev + 1)  # eof_check
+    assert e.value.code == -5
+    (2 * V).astype("<c16").tofile(tmp_path / "eigenvectors.1300.006")  # not orthonormal
+    with pytest.raises(L.LaphError) as e:
+        L.read_evectors_bin_ts(d, "eigenvectors", 1300, 6, n, nev)
+    assert e.value.code == -7
+    with open(tmp_path / "ev.005.1300", "w") as f:  # timeslice before config
+        f.write("\n".join(repr(float(x)) for x in ev) + "\n")
+    np.testing.assert_array_equal(L.read_eigenvalues_ascii(str(tmp_path / "ev"), 1300, 5, nev), ev)
+    with pytest.raises(L.LaphError) as e:
+        L.read_eigenvalues_ascii(str(tmp_path / "ev"), 1300, 5, nev + 1)
+    assert e.value.code == -5
