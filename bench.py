@@ -286,6 +286,11 @@ def workload_c2(lp, args):
     sync()
     t_batch = time.perf_counter() - t0
     evb = [b.get(t, want_vectors=False)[0] for t in range(T2)]
+    sync()
+    t0 = time.perf_counter()
+    b.source_shape(0)  # source shape of one timeslice on the device (12^3 vectors r)
+    sync()
+    t_shape = time.perf_counter() - t0
     b.close()
     lam_c = 1.35 * max(-e[-1] for e in evb)
     t_plain, dev, restarts = 0.0, 0.0, []
@@ -302,7 +307,8 @@ def workload_c2(lp, args):
     print(json.dumps({"workload": "24^3x48 eps=0.4 N_ev=120 DEG 12 tol 1e-10", "timeslices": T2,
                       "batched_ts_per_s": T2 / t_batch, "batched_cycles": st.iterations,
                       "plain_fp64_ts_per_s": T2 / t_plain, "plain_lambda_c": lam_c,
-                      "plain_restarts_max": max(restarts), "max_rel_eigenvalue_difference": dev}))
+                      "plain_restarts_max": max(restarts), "max_rel_eigenvalue_difference": dev,
+                      "source_shape_seconds_per_timeslice": t_shape}))
 
 
 def workload_c4(lp, args):

@@ -222,6 +222,14 @@ int laph_batch_solve(laph_batch* b, laph_batch_stats* stats);
  * overlapping the write of the current.  trace_sum (4 doubles, may be null)
  * receives tr(V^H V) and sum(V^H V) as (re, im, re, im). */
 int laph_batch_write(laph_batch* b, int ts, const char* dir, int config, int t, double* trace_sum);
+/* Source shape of a solved timeslice (sourceshape_funcs.cpp:31-81):
+ * psi[i] = sum_x |V_{x+r} V_x^H|_F over the ((L+1)/2)^3 vectors r of the
+ * grid {0, 2, 4, ..}^3 below L, i in the order of the reference's loops (r1,
+ * r2, r3; r3 fastest). */
+int laph_batch_source_shape(laph_batch* b, int ts, double* psi);
+/* The same for eigenvectors in host memory (nev x n, row i = eigenvector i:
+ * the eigenvectors-file layout) of an L^3 timeslice, on device `device`. */
+int laph_source_shape(int L, int nev, const double* V, int device, double* psi);
 
 /* Device time of the FP32 filter-step kernel, alone on the device: replays
  * the captured filter of the first half of the timeslices (every column

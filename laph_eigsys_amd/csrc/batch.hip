@@ -776,6 +776,15 @@ static int autotune_block_width(laph_batch* b) {
   return LAPH_OK;
 }
 
+// Phase-fixed copy of one timeslice's eigenvectors (file layout, n x nev)
+// and its phases: the output staging of laph_batch_write and
+// laph_batch_source_shape, allocated on first use.
+int output_buffers(laph_batch* b) {
+  if (!b->wV) LAPH_HIP(hipMalloc(&b->wV, (size_t)b->n * b->nev * sizeof(double2)));
+  if (!b->wph) LAPH_HIP(hipMalloc(&b->wph, b->nev * sizeof(double)));
+  return LAPH_OK;
+}
+
 static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
   auto t0 = std::chrono::steady_clock::now();
   if (!b->tuned) LAPH_TRY(autotune_block_width(b));
@@ -1005,9 +1014,8 @@ int laph_batch_write(laph_batch* b, int ts, const char* dir, int config, int t, 
   return laph::guarded([&]() -> int {
     const int n = b->n, nev = b->nev, K = b->K;
     const size_t CH = (size_t)64 << 20;  // staging chunk
-    if (!b->wV) {
-      LAPH_HIP(hipMalloc(&b->wV, (size_t)n * nev * sizeof(double2)));
-      LAPH_HIP(hipMalloc(&b->wph, nev * sizeof(double)));
+    LAPH_TRY(laph::output_buffers(b));
+    if (!b->wG) {
       LAPH_HIP(hipMalloc(&b->wG, (size_t)nev * nev * sizeof(double2)));
       LAPH_CHECK(rocblas_create_handle(&b->wrb) == rocblas_status_success &&
                      rocblas_set_stream(b->wrb, b->st) == rocblas_status_success,

@@ -103,3 +103,7 @@ struct laph_batch {
   bool force_shift = false;  // every CholQR pass starts with a shifted one (LAPH_SHIFTED_CHOLQR)
 };
 
+namespace laph {
+// Allocates laph_batch::wV / wph on first use.
+int output_buffers(laph_batch* b);
+}  // namespace laph

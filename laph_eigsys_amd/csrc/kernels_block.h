@@ -26,6 +26,10 @@ hipError_t launch_residual(const double2* X, const double2* AX, const double* la
                            void* r, double* nrm2, int n, int K, int nts, int first_col, int bw, int bf16,
                            hipStream_t st);
 // Column-major, phase-fixed copy of the first nev columns of one timeslice.
+// psi(r) = sum_x |V_{x+r} V_x^H|_F for r in {0, 2, ..}^3 below L, V in the
+// eigenvector-file layout (n x nev column-major); psi has ((L+1)/2)^3 entries
+// in the order r1, r2, r3 (r3 fastest).
+hipError_t launch_source_shape(const double2* V, int L, int nev, double* psi, hipStream_t st);
 hipError_t launch_extract_phase(const double2* X, double2* V, double* phase, int n, int K, int nev,
                                 hipStream_t st);
 // Scales every column >= first_col (a multiple of 64) of a [ts][n][K] block to

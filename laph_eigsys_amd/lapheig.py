@@ -105,6 +105,8 @@ SIGNATURES = {
     "laph_batch_get": ([_VP, _I, _dp, _dp, _dp, _dp], _I),
     "laph_batch_time_filter": ([_VP, _I, _dp, _dp], _I),
     "laph_batch_write": ([_VP, _I, _S, _I, _I, _dp], _I),
+    "laph_batch_source_shape": ([_VP, _I, _dp], _I),
+    "laph_source_shape": ([_I, _I, _dp, _I, _dp], _I),
     "laph_net_create": ([ctypes.POINTER(_VP), _I, ctypes.c_ulonglong, _I], _I),
     "laph_net_destroy": ([_VP], _I),
     "laph_net_num_params": ([_VP], _I),
@@ -252,6 +254,18 @@ def read_eigenvalues_ascii(prefix, config, t, nev):
     return out
 
 
+def source_shape(L, V, device=0):
+    """sourceshape_funcs.cpp:31-81 on the device: psi for r in range(0, L, 2)^3
+    (r3 fastest) of eigenvectors V (nev, n)."""
+    V = _c128(V)
+    nev = V.shape[0]
+    assert V.shape[1] == 3 * L ** 3
+    half = (L + 1) // 2
+    psi = np.empty(half ** 3)
+    _check(lib().laph_source_shape(L, nev, _p(V), device, _p(psi)), "laph_source_shape")
+    return psi
+
+
 # ---------------------------------------------------------------------------
 class Tslice:
     """One timeslice on the device: links, HYP, the MatShell operator and the
@@ -381,6 +395,13 @@ class Batch:
         ts_ = np.zeros(4)
         _check(lib().laph_batch_write(self.h, ts, os.fsencode(result_path), config, t, _p(ts_)), "laph_batch_write")
         return complex(ts_[0], ts_[1]), complex(ts_[2], ts_[3])
+
+    def source_shape(self, ts):
+        """psi(r) of timeslice ts of the solved batch, r in range(0, L, 2)^3."""
+        half = (self.L + 1) // 2
+        psi = np.empty(half ** 3)
+        _check(lib().laph_batch_source_shape(self.h, ts, _p(psi)), "laph_batch_source_shape")
+        return psi
 
     def get(self, ts, want_vectors=True):
         ev = np.empty(self.nev)

@@ -230,3 +230,38 @@ def gauge_transform(U, Om):
     for mu in range(3):
         out[..., mu, :, :] = dag(Om) @ U[..., mu, :, :] @ shift(Om, mu, 1)
     return out
+
+
+# ---------------------------------------------------------------------------
+# source shape (sourceshape_funcs.cpp:31-136)
+def source_shape(V, L):
+    """[((r1, r2, r3), |r|, psi)] for r in range(0, L, 2)^3 (r3 fastest):
+    psi = sum_x sqrt(tr(M M^H)), M = V_{x+r} V_x^H over the nev eigenvectors
+    (rows of V, file layout 3*site + colour)."""
+    nev = V.shape[0]
+    W = V.reshape(nev, L, L, L, 3)
+    out = []
+    for r1 in range(0, L, 2):
+        for r2 in range(0, L, 2):
+            for r3 in range(0, L, 2):
+                Wr = np.roll(W, shift=(-r1, -r2, -r3), axis=(1, 2, 3))  # W(x + r)
+                M = np.einsum("kxyza,kxyzb->xyzab", Wr, W.conj())
+                psi = np.sqrt((np.abs(M) ** 2).sum(axis=(-1, -2))).sum()
+                out.append(((r1, r2, r3), float(np.sqrt(r1 * r1 + r2 * r2 + r3 * r3)), float(psi)))
+    return out
+
+
+def avg_radii(results):
+    """sourceshape_funcs.cpp:102-136: psi averaged over equal |r| (radii equal
+    within 10e-12), in order of first appearance -> [(r, psi_avg)]."""
+    acc = []
+    for _, rad, psi in results:
+        copied = False
+        for a in acc:
+            if abs(rad - a[1]) < 10e-12:
+                a[0] += 1
+                a[2] += psi
+                copied = True
+        if not copied:
+            acc.append([1, rad, psi])
+    return [(a[1], a[2] / a[0]) for a in acc]

@@ -78,3 +78,42 @@ def test_set_phase_resets_to_stored_phases(tmp_path):
     np.testing.assert_allclose(ratio, ratio[:, :1] * np.ones((1, n)), atol=1e-12)
     np.testing.assert_array_equal(lp.read_eigenvalues_bin(str(tmp_path / "out"), "eigenvectors_old_phase",
                                                           conf, t, nev), old)
+
+
+def test_alpha_grid_follows_the_accumulating_loop():
+    """tune_hyp.cpp:116-117: a += 0.05 in double precision, a <= upper bound."""
+    from laph_eigsys_amd.tune_hyp import alpha_grid
+    g = alpha_grid(0.3, 0.7)
+    a, ref = 0.3, []
+    while a <= 0.7:
+        ref.append(a)
+        a += 0.05
+    assert g == ref
+    assert len(g) == 8 and abs(g[-1] - 0.65) < 1e-12  # 0.3 + 8 * 0.05 rounds above 0.7
+    assert alpha_grid(0.62, 0.62) == [0.62]
+    assert alpha_grid(0.5, 0.4) == []
+
+
+def test_sourceshape_host_logic(tmp_path, monkeypatch):
+    """Loop order, radius averaging and the two writers of the source-shape
+    program (read_write.cpp:206-235) against the oracle's restatement."""
+    import struct
+    import numpy as np
+    from laph_eigsys_amd import sourceshape
+    from oracle import lapheig_np as N
+    L = 4
+    psi = np.arange(8, dtype=float) + 0.5
+    res = sourceshape.results(L, psi)
+    assert [r for r, _, _ in res] == [(a, b, c) for a in (0, 2) for b in (0, 2) for c in (0, 2)]
+    avg = sourceshape.avg_radii(res)
+    assert avg == N.avg_radii(res)
+    assert avg[0] == (0.0, 0.5) and avg[1] == (2.0, (1.5 + 2.5 + 4.5) / 3)
+    monkeypatch.chdir(tmp_path)
+    fa = sourceshape.write_sourceshape_ascii("sourceshape", 1300, 7, 120, avg)
+    assert fa == "sourceshape_nev120.1300.7"
+    lines = open(fa).read().splitlines()
+    assert lines[0] == "#r\tpsi(r)" and lines[1] == "0 0.5" and len(lines) == 1 + len(avg)
+    fb = sourceshape.write_sourceshape_bin("sourceshape", 1300, 7, 120, avg)
+    assert fb == "sourceshape_nev120.1300.007.bin"
+    raw = open(fb, "rb").read()
+    assert len(raw) == 16 * len(avg) and struct.unpack("<dd", raw[16:32]) == avg[1]

@@ -110,3 +110,61 @@ def test_driver_phase_reset(gpu, tmp_path):
         iso = [i for i in range(1, nev - 1) if min(ev[i - 1] - ev[i], ev[i] - ev[i + 1]) > 1e-4]
         for i in iso:  # the same eigenvector, phase-fixed once and reset once
             np.testing.assert_allclose(Vr[i] * np.exp(-1j * old[t][i]), Vf[i], atol=1e-8)
+
+
+def test_tune_hyp_scan(gpu, tmp_path):
+    """tune_hyp: every grid point through the batched solver (one batch) and
+    the plain drop-in gives the dense oracle's spectrum of that smearing; the
+    per-point files carry the reference's names."""
+    from laph_eigsys_amd import lapheig as lp
+    from laph_eigsys_amd import tune_hyp
+    L, LT, nev, conf, ts = 6, 3, 12, 9, 1
+    cfg = N.synth_config(21, 0.4, L, LT)
+    (tmp_path / "confs").mkdir()
+    lp.write_lime(str(tmp_path / "confs" / f"conf.{conf:04d}"), cfg, LT, L)
+    vol = 72 * L ** 3
+    grid = [(a1, a2) for a1 in tune_hyp.alpha_grid(0.5, 0.6) for a2 in tune_hyp.alpha_grid(0.5, 0.55)]
+    dense = {}
+    for a1, a2 in grid:
+        U = O.hyp(O.map_timeslice(cfg[ts * vol:(ts + 1) * vol], L), L, a1, a2, 3)
+        dense[(a1, a2)] = np.sort(np.linalg.eigvalsh(N.dense_D(U.reshape(L, L, L, 3, 3, 3))))[::-1][:nev]
+    lam_c = 1.3 * max(-d[-1] for d in dense.values())
+    out = {}
+    for solver in ("batched", "plain"):
+        res = tmp_path / solver
+        (tmp_path / f"{solver}.txt").write_text(
+            f"config_path = {tmp_path / 'confs'}\nresult_path = {res}\nLT = {LT}\nLX = {L}\nNEV = {nev}\n"
+            f"alpha_1_i = 0.5\nalpha_1_f = 0.6\nalpha_2_i = 0.5\nalpha_2_f = 0.55\niter = 3\nLAM_L = 12.0\n"
+            f"LAM_C = {lam_c}\nDEG = 8\n")
+        res.mkdir()
+        pars = ev_ts.IO().set_values(str(tmp_path / f"{solver}.txt"))
+        rows = tune_hyp.tune_hyp(pars, conf, ts=ts, solver=solver, tol=1e-12 if solver == "plain" else 1e-10,
+                                 device=gpu, log=lambda m: None)
+        assert [(a1, a2) for a1, a2, _ in rows] == grid
+        for a1, a2, lo in rows:
+            ev = lp.read_eigenvalues_bin(str(res), f"eigenvalues_hyp-3_a1-{a1:.2f}_a2-{a2:.2f}", conf, ts, nev)
+            np.testing.assert_allclose(ev, dense[(a1, a2)], rtol=1e-10, atol=1e-12)
+            assert lo == ev[0]
+            ph = lp.read_eigenvalues_bin(str(res), f"phases_hyp-3_a1-{a1:.2f}_a2-{a2:.2f}", conf, ts, nev)
+            assert np.all(np.abs(ph) <= np.pi)
+        out[solver] = rows
+
+
+def test_source_shape_matches_oracle(gpu):
+    """psi(r) on the device (host eigenvectors and a solved batch) against the
+    numpy restatement of sourceshape_funcs.cpp:31-81."""
+    from laph_eigsys_amd import lapheig as lp
+    L, nev = 6, 10
+    rng = np.random.default_rng(2)
+    V = rng.standard_normal((nev, 3 * L ** 3)) + 1j * rng.standard_normal((nev, 3 * L ** 3))
+    ref = np.array([p for _, _, p in N.source_shape(V, L)])
+    np.testing.assert_allclose(lp.source_shape(L, V, device=gpu), ref, rtol=1e-12)
+    L, nev = 6, 16
+    cfg = N.synth_config(8, 0.4, L, 1)
+    b = lp.Batch(L, 8, 12.0, 0.0, 1, nev, nsub=64, tol=1e-10, seed=1, device=gpu)
+    b.set_timeslice(0, cfg, 0.62, 0.62, 3)
+    b.solve()
+    _, Vb, _, _ = b.get(0)
+    psi = b.source_shape(0)
+    b.close()
+    np.testing.assert_allclose(psi, np.array([p for _, _, p in N.source_shape(Vb, L)]), rtol=1e-12)

@@ -268,3 +268,32 @@ def test_sparse_assembly_matches_stencil():
     v = rng.normal(size=(L, L, L, 3)) + 1j * rng.normal(size=(L, L, L, 3))
     assert np.abs(A @ v.reshape(-1) + N.apply_D(U, v).reshape(-1)).max() < 1e-13
     assert abs(A - A.conj().T).max() == 0
+
+
+def test_source_shape_oracle_brute_force():
+    """The numpy source shape (roll + einsum) equals the reference's loops
+    written out literally (dll_op per site, sourceshape_funcs.cpp:31-81) on a
+    3^3 lattice, and avg_radii averages equal radii in first-appearance order."""
+    L, nev = 3, 4
+    n = 3 * L ** 3
+    rng = np.random.default_rng(1)
+    V = rng.standard_normal((nev, n)) + 1j * rng.standard_normal((nev, n))
+    got = N.source_shape(V, L)
+    idx = lambda x, y, z: ((x % L) * L + y % L) * L + z % L  # noqa: E731
+    k = 0
+    for r1 in range(0, L, 2):
+        for r2 in range(0, L, 2):
+            for r3 in range(0, L, 2):
+                psi = 0.0
+                for x in range(L):
+                    for y in range(L):
+                        for z in range(L):
+                            d, s = idx(x + r1, y + r2, z + r3), idx(x, y, z)
+                            M = V[:, 3 * d:3 * d + 3].T @ V[:, 3 * s:3 * s + 3].conj()
+                            psi += np.sqrt(np.trace(M @ M.conj().T).real)
+                assert got[k][0] == (r1, r2, r3)
+                assert abs(got[k][2] - psi) < 1e-12 * psi
+                k += 1
+    assert k == len(got) == 8
+    res = [((0, 0, 0), 0.0, 1.0), ((0, 0, 2), 2.0, 2.0), ((0, 2, 0), 2.0, 4.0), ((2, 2, 2), 12 ** 0.5, 5.0)]
+    assert N.avg_radii(res) == [(0.0, 1.0), (2.0, 3.0), (12 ** 0.5, 5.0)]
