@@ -39,11 +39,11 @@ SEED = 1227
 LAMBDA_L = 12.0
 PEAK_HBM = 8.0e12
 # HBM bytes per filter-step launch (all columns active, 32 timeslices of one
-# lane, 128-vector blocks) from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the
+# lane, 128-vector blocks, y-strips of 4) from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the
 # isolated replays of --profile-filter, each scaled by the same counter on a
 # 1 GiB device copy (python bench.py --summarize-pmc ...); None = not measured
 # for this build.
-TRAFFIC_BYTES_PER_LAUNCH = 38821704728  # profiles/filter_step_32x32x32x64_k256_pmc.md
+TRAFFIC_BYTES_PER_LAUNCH = 29074033011  # profiles/filter_step_32x32x32x64_k256_pmc.md
 
 
 def main():
