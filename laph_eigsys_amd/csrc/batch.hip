@@ -278,6 +278,37 @@ static int cholqr(const View& v, Act q, bool shifted = false) {
 // U^{-H} Mt U^{-1} w = mu w, z = U^{-1} w.  mu ascending = eigenvalues of -D
 // nearest 0 first.  New block: Xc_A = Z^H Yc_A; AX_A = D X_A by the FP64
 // blocked stencil (bandwidth-bound, far cheaper than rotating AY_A by Z).
+// Equilibration of the generalized Rayleigh-Ritz without a pass over the
+// block: s_i = 1 / sqrt(Gt_ii) (the inverse column norms), Gt and Mt scaled
+// to S Gt S, S Mt S (the Gram and projection of the unit-column block Y S),
+// and the rotation of Y S by Z applied to Y as S Z.
+__global__ void k_equil_scale(const double2* __restrict__ Gt, double* __restrict__ sc, int a, int K) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a) sc[(size_t)blockIdx.y * K + i] = 1.0 / sqrt(Gt[(size_t)blockIdx.y * K * K + (size_t)i * K + i].x);
+}
+__global__ void k_equil_apply(double2* __restrict__ A, double2* __restrict__ B, const double* __restrict__ sc, int a,
+                              int K) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a * a) return;
+  const int i = idx % a, j = idx / a;
+  const double* s = sc + (size_t)blockIdx.y * K;
+  const double f = s[i] * s[j];
+  const size_t e = (size_t)blockIdx.y * K * K + (size_t)j * K + i;
+  A[e].x *= f;
+  A[e].y *= f;
+  B[e].x *= f;
+  B[e].y *= f;
+}
+__global__ void k_scale_rows(double2* __restrict__ Z, const double* __restrict__ sc, int a, int K) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a * a) return;
+  const int i = idx % a, j = idx / a;
+  const double f = sc[(size_t)blockIdx.y * K + i];
+  const size_t e = (size_t)blockIdx.y * K * K + (size_t)j * K + i;
+  Z[e].x *= f;
+  Z[e].y *= f;
+}
+
 static int rayleigh_ritz(const View& v, Act q, bool orthonormal, bool* chol_failed) {
   laph_batch* b = v.b;
   const int K = b->K, nt = v.ln->nt;
@@ -292,8 +323,13 @@ static int rayleigh_ritz(const View& v, Act q, bool orthonormal, bool* chol_fail
   rocblas_handle rb = v.ln->rb;
   *chol_failed = false;
   LAPH_TRY(gram(v, q, AY, Y, G, -1.0));  // Mt
+  double* sc = b->cn + v.nk();
+  const dim3 gs((unsigned)((q.a + 255) / 256), nt), ga((unsigned)((q.a * q.a + 255) / 256), nt);
   if (!orthonormal) {
     LAPH_TRY(gram(v, q, Y, Y, G2, 1.0));  // Gt
+    k_equil_scale<<<gs, 256, 0, v.ln->st>>>(G2, sc, q.a, K);
+    k_equil_apply<<<ga, 256, 0, v.ln->st>>>(G2, G, sc, q.a, K);
+    LAPH_HIP(hipGetLastError());
     LAPH_RB(rocsolver_zpotrf_strided_batched(rb, rocblas_fill_upper, q.a, zc(G2), K, kk, info, nt));
     if (dense_sync_info(v, "Rayleigh-Ritz (zpotrf)") != LAPH_OK) {
       *chol_failed = true;
@@ -317,9 +353,12 @@ static int rayleigh_ritz(const View& v, Act q, bool orthonormal, bool* chol_fail
                                              b->mu + v.nk() + q.l, K, b->E + v.nk(), K, info, nt));
   }
   LAPH_TRY(dense_sync_info(v, "Rayleigh-Ritz (zheevd)"));
-  if (!orthonormal)
+  if (!orthonormal) {
     LAPH_RB(rocblas_ztrsm_strided_batched(rb, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
                                           rocblas_diagonal_non_unit, q.a, q.a, &one, zc(G2), K, kk, zc(G), K, kk, nt));
+    k_scale_rows<<<ga, 256, 0, v.ln->st>>>(G, sc, q.a, K);
+    LAPH_HIP(hipGetLastError());
+  }
   LAPH_RB(rocblas_zgemm_strided_batched(rb, rocblas_operation_conjugate_transpose, rocblas_operation_none, q.a, b->n,
                                         q.a, &one, zc(G), K, kk, zc(Y + q.l), K, (rocblas_stride)b->per_ts, &zero,
                                         zc(T + q.l), K, (rocblas_stride)b->per_ts, nt));
@@ -344,10 +383,6 @@ static int orthonormalise_and_rr(const View& v, Act q, bool explicit_qr) {
     if (explicit_qr || pass == 1) {
       LAPH_TRY(cholqr(v, q));
       LAPH_TRY(cholqr(v, q));
-    } else {
-      // unit columns: equilibrates the Gram whose Cholesky the generalized
-      // Rayleigh-Ritz takes (the span is unchanged)
-      LAPH_HIP(launch_normalize_columns(v.X(), b->cn + v.nk(), b->n, b->K, v.ln->nt, q.l, v.ln->st));
     }
     LAPH_HIP(launch_lap_block64(b->U64 + v.lo(), v.X(), v.AX(), b->p.L, b->K, v.ln->nt, b->lock_blk + v.ln->t0,
                                 b->strip, v.ln->st));
