@@ -169,6 +169,17 @@ int laph_synth_config(unsigned long long seed, double eps, int L, int T, double*
  * ev_ts.cpp:96-126) for the EPS solve + fix_phase + recover_spectrum part. */
 typedef struct laph_batch laph_batch;
 
+/* Environment controls of a batch, read by laph_batch_create (for tuning and
+ * tests; the defaults are the measured best):
+ *   LAPH_LANES=1..4         host lanes (threads, streams) the timeslices are split over (2)
+ *   LAPH_BW=64|128|256      vectors per filter block (else timed on the first solve)
+ *   LAPH_STRIP=w            sweep strip width of the stencils, w divides L (else timed)
+ *   LAPH_WARM=n             single-pass cycles before T_m(B)^P cycles (2)
+ *   LAPH_QR_CYCLES=n        cycles (and start block) orthonormalised by explicit CholQR2 (0)
+ *   LAPH_SHIFTED_CHOLQR=1   every CholQR pass starts with a shifted one
+ *   LAPH_EIG=0|1|2          Rayleigh-Ritz eigensolver: zheevd, zheevdj, zheevj (0)
+ *   LAPH_TRACE=1            per-cycle trace of the first timeslice on stderr */
+
 typedef struct laph_batch_opts {
   int nts;           /* timeslices held by the batch */
   int nev;           /* eigenpairs per timeslice */
@@ -251,7 +262,7 @@ int laph_batch_get(laph_batch* b, int ts, double* evals, double* V_fix, double* 
  * arithmetic) that maps Gaussian colour vectors to the starting block of the
  * batched solver.  No counterpart in the reference (its EPS starts from
  * SLEPc's random vector); the random start stays the fallback: a batch
- * solved after laph_net_propose is judged by the same residual test. */
+ * solved after laph_net_propose is checked by the same residual test. */
 typedef struct laph_net laph_net;
 
 int laph_net_create(laph_net** out, int layers, unsigned long long seed, int device);

@@ -11,7 +11,8 @@ It also holds the definition of the synthetic gauge field:
   U = exp(i eps H),  H = sum_{a=1..8} xi_a lambda_a / 2,  xi_a ~ N(0,1),
 with the Gaussians drawn from a counter-based hash keyed by
 (seed, t, x, y, z, mu) -- the same definition the product's generator
-implements (laph_eigsys_amd/csrc/synth.cpp), so both can be compared.
+implements (laph_synth_config in laph_eigsys_amd/csrc/host_io.cpp), so both can
+be compared.
 """
 import numpy as np
 

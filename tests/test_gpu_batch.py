@@ -152,3 +152,21 @@ def test_shifted_cholqr_path(lp, gpu, monkeypatch):
         ev, V, ph, res = b.get(t)
         check_pairs(ev, V, res, N.dense_D(U.reshape(L, L, L, 3, 3, 3)), nev, tol)
     b.close()
+
+
+@pytest.mark.parametrize("L,nts,nev,nsub", [(5, 3, 16, 64), (7, 1, 30, 64), (9, 2, 60, 64)])
+def test_batch_edge_shapes(lp, gpu, L, nts, nev, nsub):
+    """Odd L and L not divisible by 4 (plain sweep order, partial last
+    site-block), a single timeslice (one lane), three timeslices over two
+    lanes, and nev close to the search-space width."""
+    tol = 1e-10
+    data = smeared(L, 4000 + L, 0.4, nts)
+    b = lp.Batch(L, 10, 12.0, 0.0, nts, nev, nsub=nsub, tol=tol, seed=3, device=gpu)
+    for t, (ts, _) in enumerate(data):
+        b.set_timeslice(t, ts, 0.62, 0.62, 3)
+    st = b.solve()
+    assert st.converged == nts and st.filter_strip == L
+    for t, (_, U) in enumerate(data):
+        ev, V, ph, res = b.get(t)
+        check_pairs(ev, V, res, N.dense_D(U.reshape(L, L, L, 3, 3, 3)), nev, tol)
+    b.close()

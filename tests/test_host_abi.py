@@ -153,3 +153,17 @@ def test_readers_round_trip_and_errors(L, tmp_path):
     with pytest.raises(L.LaphError) as e:
         L.read_eigenvalues_ascii(str(tmp_path / "ev"), 1300, 5, nev + 1)
     assert e.value.code == -5
+
+
+@pytest.mark.parametrize("kw", [dict(L=2), dict(deg=1), dict(lambda_l=0.0), dict(nts=0), dict(nev=0),
+                                dict(tol=0.0), dict(filter_passes=7), dict(filter_storage=2),
+                                dict(L=3, nev=20, nsub=128), dict(nev=64, nsub=64)])
+def test_batch_rejects_bad_arguments(L, kw):
+    """laph_batch_create validates its arguments before touching a device:
+    each bad one is LAPH_ERR_INVALID with a message (no GPU needed)."""
+    a = dict(L=6, deg=8, lambda_l=12.0, nts=2, nev=10, nsub=0, tol=1e-10, filter_passes=0, filter_storage=0)
+    a.update(kw)
+    with pytest.raises(L.LaphError) as e:
+        L.Batch(a["L"], a["deg"], a["lambda_l"], 0.0, a["nts"], a["nev"], nsub=a["nsub"], tol=a["tol"],
+                filter_passes=a["filter_passes"], filter_storage=a["filter_storage"])
+    assert e.value.code == -1 and "laph_batch_create" in str(e.value)
