@@ -41,6 +41,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <string>
 #include <vector>
 
 #include "common.h"
@@ -260,7 +261,24 @@ static int cholqr(const View& v, Act q, bool shifted = false) {
   LAPH_RB(rocsolver_zpotrf_strided_batched(v.ln->rb, rocblas_fill_upper, q.a, zc(G), K, (rocblas_stride)K * K,
                                            v.b->info + v.ln->t0, v.ln->nt));
   if (dense_sync_info(v, "CholQR (zpotrf)") != LAPH_OK) {
-    if (shifted) return LAPH_ERR_NOCONV;  // the message names the failing timeslice
+    if (shifted) {  // the shifted Gram is definite for any finite block: report what the block holds
+      const std::string msg = laph_last_error();
+      std::vector<double> cn((size_t)v.ln->nt * K);
+      LAPH_HIP(hipMemcpy(cn.data(), v.b->cn + v.nk(), cn.size() * sizeof(double), hipMemcpyDeviceToHost));
+      int bad = 0;
+      double lo = INFINITY, hi = 0.0;
+      for (int t = 0; t < v.ln->nt; ++t)
+        for (int j = q.l; j < K; ++j) {
+          const double c = cn[(size_t)t * K + j];
+          if (!std::isfinite(c)) ++bad;
+          else {
+            lo = std::min(lo, c);
+            hi = std::max(hi, c);
+          }
+        }
+      return set_error(LAPH_ERR_NOCONV, "%s; shifted pass failed too: %d non-finite column norms, finite ones in "
+                       "[%.3g, %.3g]", msg.c_str(), bad, lo, hi);
+    }
     LAPH_TRY(cholqr(v, q, true));
     return cholqr(v, q);
   }
@@ -657,6 +675,20 @@ static int filter_passes(const laph_batch* b, const std::vector<double>& lc) {
 }
 
 // Solve loop of one lane (runs on its own host thread).
+// Largest residual of the wanted pairs not yet converged over the lane's
+// timeslices (0 when all are).
+static double worst_residual(const laph_batch* b, const laph_lane* ln) {
+  double w = 0.0;
+  for (int t = 0; t < ln->nt; ++t) {
+    const double* res = &b->h_res[(size_t)(ln->t0 + t) * b->K];
+    for (int j = 0; j < b->nev; ++j) {
+      const double r = std::sqrt(res[j]);
+      if (r > b->o.tol) w = std::max(w, r);
+    }
+  }
+  return w;
+}
+
 static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
   const View v{b, ln};
   const int K = b->K, nt = ln->nt;
@@ -728,11 +760,12 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
     LAPH_HIP(hipEventRecord(ln->ev[2], ln->st));
     LAPH_TRY(residuals_and_window(v, lc, &active));
     if (b->trace && ln->t0 == 0) {  // per-cycle trace of the first timeslice
+      const double worst = worst_residual(b, ln);
       const double* mu = &b->h_mu[0];
       const double* rs = &b->h_res[0];
-      std::fprintf(stderr, "[laph] cycle %d: window %.4f  mu[nev-1] %.5f  mu[K-1] %.4f  res[0] %.2e  res[nev-1] %.2e  "
-                   "locked %d  lock %d\n", it, lc[0], mu[b->nev - 1], mu[K - 1], std::sqrt(rs[0]),
-                   std::sqrt(rs[b->nev - 1]), b->nlock[0], ln->lock);
+      std::fprintf(stderr, "[laph] cycle %d: passes %d  window %.4f  mu[nev-1] %.5f  mu[K-1] %.4f  res[0] %.2e  "
+                   "res[nev-1] %.2e  worst %.2e  locked %d  lock %d\n", it, passes, lc[0], mu[b->nev - 1], mu[K - 1],
+                   std::sqrt(rs[0]), std::sqrt(rs[b->nev - 1]), worst, b->nlock[0], ln->lock);
     }
     float a = 0, c = 0;
     LAPH_HIP(hipEventElapsedTime(&a, ln->ev[0], ln->ev[1]));
