@@ -178,7 +178,9 @@ typedef struct laph_batch laph_batch;
  *   LAPH_QR_CYCLES=n        cycles (and start block) orthonormalised by explicit CholQR2 (0)
  *   LAPH_SHIFTED_CHOLQR=1   every CholQR pass starts with a shifted one
  *   LAPH_EIG=0|1|2          Rayleigh-Ritz eigensolver: zheevd, zheevdj, zheevj (0)
- *   LAPH_TRACE=1            per-cycle trace of the first timeslice on stderr */
+ *   LAPH_TRACE=1            per-cycle trace of the first timeslice on stderr
+ *   LAPH_TEST_POISON=t      (tests) NaN written into one row of timeslice t's unlocked columns in
+ *                           cycle 2, to exercise the recovery counted in laph_batch_stats */
 
 typedef struct laph_batch_opts {
   int nts;           /* timeslices held by the batch */
@@ -213,6 +215,8 @@ typedef struct laph_batch_stats {
   int filter_strip;        /* sweep strip width of the stencil kernels (L = x-major
                               planes, or 4), chosen with the block width unless
                               LAPH_STRIP fixes it */
+  int recoveries;          /* times the active columns of failing timeslices were re-seeded
+                              after even shifted Cholesky QR failed (a non-finite block) */
 } laph_batch_stats;
 
 int laph_batch_create(laph_batch** out, const laph_params* params, const laph_batch_opts* opts);

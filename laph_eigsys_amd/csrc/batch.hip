@@ -399,7 +399,17 @@ static int orthonormalise_and_rr(const View& v, Act q, bool explicit_qr) {
   laph_batch* b = v.b;
   for (int pass = 0; pass < 2; ++pass) {
     if (explicit_qr || pass == 1) {
-      LAPH_TRY(cholqr(v, q));
+      int rc = cholqr(v, q);
+      if (rc == LAPH_ERR_NOCONV) {
+        // even the shifted pass failed (a block that is not finite): the
+        // failing timeslices (non-zero info) restart their active columns
+        // from Gaussian vectors; the others keep theirs
+        if (b->trace) std::fprintf(stderr, "[laph] lane %d: %s; active columns re-seeded\n", v.ln->t0, laph_last_error());
+        LAPH_HIP(launch_reseed_cols(v.X(), b->n, b->K, v.ln->nt, q.l, b->info + v.ln->t0,
+                                    b->o.seed + 0x5851F42Dull * (++v.ln->recoveries), v.ln->st));
+        rc = cholqr(v, q);
+      }
+      LAPH_TRY(rc);
       LAPH_TRY(cholqr(v, q));
     }
     LAPH_HIP(launch_lap_block64(b->U64 + v.lo(), v.X(), v.AX(), b->p.L, b->K, v.ln->nt, b->lock_blk + v.ln->t0,
@@ -698,6 +708,7 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
   ln->lock = 0;
   ln->cycles = 0;
   ln->sv32 = ln->sv64 = ln->flaunch = 0;
+  ln->recoveries = 0;
   ln->fbytes = 0;
   ln->ms_filter = ln->ms_dense = 0;
   {
@@ -753,6 +764,9 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
                                                     b->per_ts, K, b->first_blk + ln->t0, b->bw);
     LAPH_HIP(hipGetLastError());
     const Act q{ln->lock, K - ln->lock};
+    if (it == 2 && b->poison_ts >= ln->t0 && b->poison_ts < ln->t0 + nt)  // LAPH_TEST_POISON (tests)
+      LAPH_HIP(hipMemsetAsync(v.X() + (size_t)(b->poison_ts - ln->t0) * b->per_ts + 7 * (size_t)K + q.l, 0xFF,
+                              (size_t)q.a * sizeof(double2), ln->st));
     LAPH_TRY(project_locked(v, q));
     // explicit CholQR2 while the block is far from orthonormal (first
     // cycles from a random start), Cholesky inside Rayleigh-Ritz after
@@ -888,6 +902,7 @@ static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
     s.stencil_vecs += ln.sv32;
     s.stencil64_vecs += ln.sv64;
     s.filter_launches += ln.flaunch;
+    s.recoveries += ln.recoveries;
     s.filter_bytes += ln.fbytes;
     s.seconds_filter += ln.ms_filter * 1e-3;
     s.seconds_dense += ln.ms_dense * 1e-3;
@@ -949,6 +964,7 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     if (const char* e = std::getenv("LAPH_WARM")) b->warm_cycles = std::max(0, std::atoi(e));
     b->force_shift = std::getenv("LAPH_SHIFTED_CHOLQR") != nullptr;
     if (const char* e = std::getenv("LAPH_QR_CYCLES")) b->qr_cycles = std::max(0, std::atoi(e));
+    if (const char* e = std::getenv("LAPH_TEST_POISON")) b->poison_ts = std::atoi(e);
     b->strip = strip > 0 ? strip : p->L;
     b->strip_fixed = strip > 0;
     b->bf16 = o->filter_storage == 1;

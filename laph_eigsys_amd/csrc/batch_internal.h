@@ -36,6 +36,7 @@ struct laph_lane {
   int cycles = 0, rc = 0;
   std::string msg;  // error message raised on the lane's thread
   long long sv32 = 0, sv64 = 0, flaunch = 0;
+  int recoveries = 0;  // timeslice blocks re-seeded after a failed orthonormalisation
   double fbytes = 0;
   float ms_filter = 0, ms_dense = 0;
 };
@@ -91,6 +92,7 @@ struct laph_batch {
   int strip = 0;  // sweep strip width of the stencil kernels (divides L; L = x-major planes)
   bool strip_fixed = false;  // LAPH_STRIP given
   bool tuned = false;        // filter layout chosen (first solve)
+  int poison_ts = -1;  // LAPH_TEST_POISON: NaN into one row of this timeslice's active block at cycle 2 (tests)
   int qr_cycles = 0;  // cycles (and the start block) orthonormalised by explicit CholQR2 (LAPH_QR_CYCLES)
   // laph_batch_write: phase-fixed copy, phases, V^H V, rocBLAS handle on st,
   // two pinned staging chunks and their events (created on first use)

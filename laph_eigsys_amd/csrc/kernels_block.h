@@ -37,5 +37,8 @@ hipError_t launch_extract_phase(const double2* X, double2* V, double* phase, int
 hipError_t launch_normalize_columns(double2* Y, double* tmp, int n, int K, int nts, int first_col, hipStream_t st);
 hipError_t launch_convert_links32(const double2* U, float2* U32, size_t n, hipStream_t st);
 hipError_t launch_random_block(double2* X, size_t total, unsigned long long seed, hipStream_t st);
+// Gaussian columns [first_col, K) of the timeslices with flag[ts] != 0.
+hipError_t launch_reseed_cols(double2* X, int n, int K, int nts, int first_col, const int* flag,
+                              unsigned long long seed, hipStream_t st);
 
 }  // namespace laph

@@ -534,6 +534,30 @@ __global__ void k_random_block(double2* __restrict__ X, size_t total, unsigned l
   X[i] = make_double2(rr * cos(2.0 * M_PI * u2), rr * sin(2.0 * M_PI * u2));
 }
 
+// Gaussian entries in columns [first_col, K) of the timeslices whose flag is
+// non-zero (X[ts][row][K]); the others are left alone.
+__global__ void k_reseed_cols(double2* __restrict__ X, int n, int K, int first_col, const int* __restrict__ flag,
+                              unsigned long long seed) {
+  const int ts = blockIdx.y;
+  if (flag[ts] == 0) return;
+  const int w = K - first_col;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)n * w) return;
+  const size_t row = i / w, col = first_col + i % w;
+  auto mix = [](unsigned long long z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  const unsigned long long c = ((unsigned long long)ts * n + row) * K + col;
+  const unsigned long long h1 = mix(seed * 0x9E3779B97F4A7C15ull + (2 * c + 1) * 0xD1B54A32D192ED03ull);
+  const unsigned long long h2 = mix(seed * 0x9E3779B97F4A7C15ull + (2 * c + 2) * 0xD1B54A32D192ED03ull);
+  const double u1 = ((double)(h1 >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  const double u2 = ((double)(h2 >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  const double rr = sqrt(-2.0 * log(u1));
+  X[(size_t)ts * n * K + row * K + col] = make_double2(rr * cos(2.0 * M_PI * u2), rr * sin(2.0 * M_PI * u2));
+}
+
 // ---------------------------------------------------------------------------
 static inline dim3 site_grid(int L, int K, int nts) {
   const int V3 = L * L * L;
@@ -604,6 +628,13 @@ hipError_t launch_extract_phase(const double2* X, double2* V, double* phase, int
 
 hipError_t launch_convert_links32(const double2* U, float2* U32, size_t n, hipStream_t st) {
   k_convert_links32<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(U, U32, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_reseed_cols(double2* X, int n, int K, int nts, int first_col, const int* flag,
+                              unsigned long long seed, hipStream_t st) {
+  const size_t total = (size_t)n * (K - first_col);
+  k_reseed_cols<<<dim3((unsigned)((total + 255) / 256), nts), 256, 0, st>>>(X, n, K, first_col, flag, seed);
   return hipGetLastError();
 }
 

@@ -170,3 +170,22 @@ def test_batch_edge_shapes(lp, gpu, L, nts, nev, nsub):
         ev, V, ph, res = b.get(t)
         check_pairs(ev, V, res, N.dense_D(U.reshape(L, L, L, 3, 3, 3)), nev, tol)
     b.close()
+
+
+def test_recovery_from_a_non_finite_block(lp, gpu, monkeypatch):
+    """A timeslice whose unlocked columns turn non-finite (NaN written into one
+    row by the LAPH_TEST_POISON control) makes every Cholesky fail; its active
+    columns are re-seeded, the batch still converges, and every pair matches
+    the dense oracle."""
+    L, nts, nev, tol = 6, 2, 20, 1e-10
+    data = smeared(L, 606, 0.4, nts)
+    monkeypatch.setenv("LAPH_TEST_POISON", "1")
+    b = lp.Batch(L, 8, 12.0, 0.0, nts, nev, nsub=64, tol=tol, seed=6, device=gpu)
+    for t, (ts, _) in enumerate(data):
+        b.set_timeslice(t, ts, 0.62, 0.62, 3)
+    st = b.solve()
+    assert st.recoveries >= 1 and st.converged == nts
+    for t, (_, U) in enumerate(data):
+        ev, V, ph, res = b.get(t)
+        check_pairs(ev, V, res, N.dense_D(U.reshape(L, L, L, 3, 3, 3)), nev, tol)
+    b.close()
