@@ -470,6 +470,67 @@ static int residuals_and_window(const View& v, std::vector<double>& lc, int* n_a
 
 // Per-column coefficients of the filter steps of a cycle of `passes`
 // applications of T_m(B), and the combine factor gamma^passes.
+// Roots of T_m, z_j = cos((2j-1) pi / 2m), in Leja order: each next root
+// maximises the product of its distances to those already taken, starting
+// from the largest.  Applied in this order the partial products
+// prod_{j<=k} beta_j (x - z_j) stay within a small factor of 1 on [-1, 1].
+static std::vector<double> leja_roots(int m) {
+  std::vector<double> z(m), o;
+  for (int j = 0; j < m; ++j) z[j] = std::cos((2.0 * j + 1.0) * M_PI / (2.0 * m));
+  std::vector<bool> used(m, false);
+  for (int k = 0; k < m; ++k) {
+    int best = -1;
+    double bv = -INFINITY;
+    for (int i = 0; i < m; ++i) {
+      if (used[i]) continue;
+      double lp = 0.0;
+      for (double q : o) lp += std::log(std::fabs(z[i] - q));
+      if (k == 0) lp = std::fabs(z[i]);
+      if (lp > bv) bv = lp, best = i;
+    }
+    used[best] = true;
+    o.push_back(z[best]);
+  }
+  return o;
+}
+
+// Product-form coefficients of one column with Ritz value th (in B):
+// T_m(x) = prod_k beta_k (x - z_k), beta_1 = 1, beta_k = 2, and p_k the
+// partial products, s_k = max(|p_k(th)|, 1).
+//   correction: e_k = beta_k (B - z_k) e_{k-1} + beta_k p_{k-1}(th) r, e_1 = r;
+//               step st makes e_{st+1} / s_{st+1} from e_st / s_st (st = 1
+//               reads e_1 = r unscaled);
+//   plain pass: h_k = (beta_k s_{k-1} / s_k) (B - z_k) h_{k-1}, h_0 = w, and
+//               the last step adds gamma^(p-1) f_m.
+// co / co2 point at this column's entry of the first step; steps are
+// `stride` apart.  *g = gamma^passes with gamma = T_m(th) / s_m.
+// pk and sk are scratch arrays of m + 1 entries.
+static void product_coefficients(const laph_batch* b, double th, int m, float4* co, float4* co2, size_t stride,
+                                 double* g, int passes, double* pk, double* sk) {
+  const std::vector<double>& z = b->roots;
+  pk[0] = 1.0;
+  sk[0] = 1.0;
+  for (int k = 1; k <= m; ++k) {
+    pk[k] = (k == 1 ? 1.0 : 2.0) * (th - z[k - 1]) * pk[k - 1];
+    sk[k] = std::max(std::fabs(pk[k]), 1.0);
+  }
+  auto step = [&](int k, double in_scale, double rcoef) {  // makes index k from k-1
+    const double bk = k == 1 ? 1.0 : 2.0;
+    const double a1 = bk * in_scale / sk[k];
+    return make_float4((float)a1, (float)rcoef, 0.f, (float)(-a1 * z[k - 1]));
+  };
+  for (int st = 1; st < m; ++st) {
+    const int k = st + 1;
+    co[(size_t)(st - 1) * stride] = step(k, st == 1 ? 1.0 : sk[k - 1], 2.0 * pk[k - 1] / sk[k]);
+  }
+  const double gam = pk[m] / sk[m];
+  for (int st = 1; st < m; ++st) co2[(size_t)(st - 1) * stride] = step(st, sk[st - 1], 0.0);
+  double gp = gam;
+  for (int pp = 2; pp <= LAPH_MAX_PASSES; ++pp, gp *= gam)
+    co2[(size_t)(m - 1 + pp - 2) * stride] = step(m, sk[m - 1], gp);
+  *g = std::pow(gam, passes);
+}
+
 static int upload_coefficients(const View& v, const std::vector<double>& lc, int passes) {
   laph_batch* b = v.b;
   laph_lane* ln = v.ln;
@@ -490,6 +551,11 @@ static int upload_coefficients(const View& v, const std::vector<double>& lc, int
     const double* mu = &b->h_mu[v.nk() + (size_t)t * K];
     for (int j = 0; j < K; ++j) {
       const double th = 1.0 + c * (lc[t] - mu[j]);  // Ritz value in B
+      if (b->prod) {
+        product_coefficients(b, th, m, &co[(size_t)t * K + j], &co2[(size_t)t * K + j], (size_t)nt * K,
+                             &ga[(size_t)t * K + j], passes, tau.data(), s.data());
+        continue;
+      }
       tau[0] = 1.0;
       tau[1] = th;
       for (int k = 1; k < m; ++k) tau[k + 1] = 2.0 * th * tau[k] - tau[k - 1];
@@ -541,10 +607,11 @@ static int enqueue_filter(const View& v) {
   const size_t step = (size_t)b->nts * K;
   const size_t vo = v.vo();
   void* R = fb(b, b->R32, vo);
+  const int nb = b->prod ? 2 : 3;  // the product form needs no previous vector
   for (int st = 1; st < m; ++st) {
-    const void* in = st == 1 ? R : fb(b, b->P[(st - 2) % 3], vo);
-    const void* old = st >= 3 ? fb(b, b->P[(st - 3) % 3], vo) : nullptr;
-    void* out = fb(b, b->P[(st - 1) % 3], vo);
+    const void* in = st == 1 ? R : fb(b, b->P[(st - 2) % nb], vo);
+    const void* old = st >= 3 && !b->prod ? fb(b, b->P[(st - 3) % 3], vo) : nullptr;
+    void* out = fb(b, b->P[(st - 1) % nb], vo);
     LAPH_HIP(launch_filter_step(b->U32 + v.lo(), in, R, old, out, b->coef + (st - 1) * step + v.nk(),
                                 b->cab + v.ln->t0, b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, b->strip, v.ln->st));
   }
@@ -558,15 +625,32 @@ static void plain_buffers(const View& v, void** pf, void* q[3]) {
   laph_batch* b = v.b;
   const int m = b->deg;
   void* all[4] = {b->R32, b->P[0], b->P[1], b->P[2]};
-  void* f = b->P[(m - 2) % 3];
+  void* f = b->P[(m - 2) % (b->prod ? 2 : 3)];
   int k = 0;
   for (void* a : all)
     if (a != f) q[k++] = fb(b, a, v.vo());
   *pf = fb(b, f, v.vo());
 }
 
+// Product form: the two slots of q[] pass p >= 2 writes, alternately (pass 2
+// reads f, a later pass the slot where the previous one ended).
+static void product_outputs(int m, int p, int o[2]) {
+  int a = -1;
+  for (int k = 2; k <= p; ++k) {
+    o[0] = k == 2 ? 0 : (a + 1) % 3;
+    o[1] = k == 2 ? 1 : (a + 2) % 3;
+    a = o[(m - 1) % 2];
+  }
+}
+
 // Slot of q[] holding w_p after plain pass p >= 2.
-static int plain_result(int m, int p) {
+static int plain_result(const laph_batch* b, int p) {
+  const int m = b->deg;
+  if (b->prod) {
+    int o[2];
+    product_outputs(m, p, o);
+    return o[(m - 1) % 2];
+  }
   int r = (m - 1) % 3;
   for (int k = 3; k <= p; ++k) r = (r + m) % 3;
   return r;
@@ -582,11 +666,17 @@ static int enqueue_plain_pass(const View& v, int p) {
   const size_t step = (size_t)b->nts * K;
   void *f, *q[3];
   plain_buffers(v, &f, q);
-  const int a = p == 2 ? 0 : plain_result(m, p - 1);
+  const int a = p == 2 ? 0 : plain_result(b, p - 1);
+  int po[2] = {0, 1};
+  if (b->prod) product_outputs(m, p, po);
   for (int st = 1; st <= m; ++st) {
     const void *in, *old;
     void* out;
-    if (p == 2) {
+    if (b->prod) {
+      in = st == 1 ? (p == 2 ? f : q[a]) : q[po[(st - 2) % 2]];
+      old = nullptr;
+      out = q[po[(st - 1) % 2]];
+    } else if (p == 2) {
       in = st == 1 ? f : q[(st - 2) % 3];
       old = st == 1 ? nullptr : (st == 2 ? f : q[(st - 3) % 3]);
       out = q[(st - 1) % 3];
@@ -630,7 +720,7 @@ static int run_filter(const View& v, int passes, void** w) {
   *w = f;
   for (int p = 2; p <= passes; ++p) {
     LAPH_HIP(hipGraphLaunch(ln->gplain[p - 2], ln->st));
-    *w = q[plain_result(v.b->deg, p)];
+    *w = q[plain_result(v.b, p)];
   }
   return LAPH_OK;
 }
@@ -739,7 +829,8 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
       // A vector touch is 3 colours x esz bytes per site; links 27 x 8 B per
       // site per vector block
       for (int s = 1; s < b->deg; ++s) {
-        const int touches = s == 1 ? 2 : (s == 2 ? 3 : 4);
+        // product form: read in and r, write out; three-term: also the previous vector
+      const int touches = s == 1 ? 2 : (s == 2 || b->prod ? 3 : 4);
         ln->fbytes += (double)b->V3 * ((double)cols * touches * 3.0 * b->esz + nblk * 216.0);
       }
       for (int p = 2; p <= passes; ++p) {
@@ -965,6 +1056,8 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     b->force_shift = std::getenv("LAPH_SHIFTED_CHOLQR") != nullptr;
     if (const char* e = std::getenv("LAPH_QR_CYCLES")) b->qr_cycles = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("LAPH_TEST_POISON")) b->poison_ts = std::atoi(e);
+    if (const char* e = std::getenv("LAPH_FILTER_FORM")) b->prod = std::atoi(e) != 0;
+    b->roots = laph::leja_roots(p->deg);
     b->strip = strip > 0 ? strip : p->L;
     b->strip_fixed = strip > 0;
     b->bf16 = o->filter_storage == 1;

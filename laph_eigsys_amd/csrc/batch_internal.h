@@ -102,6 +102,9 @@ struct laph_batch {
   rocblas_handle wrb = nullptr;
   void* pin[2] = {nullptr, nullptr};
   hipEvent_t wev[2] = {nullptr, nullptr};
+  bool prod = true;          // product-form filter over Leja-ordered roots of T_m; LAPH_FILTER_FORM=0:
+                             // three-term recurrence
+  std::vector<double> roots; // roots of T_deg in the order the product form applies them
   bool force_shift = false;  // every CholQR pass starts with a shifted one (LAPH_SHIFTED_CHOLQR)
 };
 

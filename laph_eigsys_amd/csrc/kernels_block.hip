@@ -143,9 +143,10 @@ __device__ __forceinline__ void hop(const C* __restrict__ U, const C* __restrict
 }
 
 // One Chebyshev step in FP32 for all timeslices:
-//   out = a1 * (B in) + a2 * r + b * old,   B = c_t (H - 6) + a_t
-// with per-column coefficients (a1, a2, b) for this step; HAS_R = false is
-// the plain recurrence (no source term, one vector stream less).
+//   out = a1 * (B in) + a3 * in + a2 * r + b * old,   B = c_t (H - 6) + a_t
+// with per-column coefficients (a1, a2, b, a3) for this step; HAS_R = false
+// is the plain recurrence (no source term, one vector stream less), and a
+// null old the product-form step (no previous vector).
 template <bool HAS_R>
 __global__ __launch_bounds__(256) void k_filter_step32(const float2* __restrict__ U32,
                                                         const float2* __restrict__ in,
@@ -180,8 +181,8 @@ __global__ __launch_bounds__(256) void k_filter_step32(const float2* __restrict_
     bv.x = ca.x * acc[c].x + ca.y * ctr[c].x;
     bv.y = ca.x * acc[c].y + ca.y * ctr[c].y;
     float2 o;
-    o.x = cf.x * bv.x;
-    o.y = cf.x * bv.y;
+    o.x = cf.x * bv.x + cf.w * ctr[c].x;
+    o.y = cf.x * bv.y + cf.w * ctr[c].y;
     if (HAS_R) {
       const float2 rv = ld_nt(r + idx);
       o.x += cf.y * rv.x;
@@ -265,10 +266,10 @@ __global__ __launch_bounds__(256) void k_filter_step32x2(const float2* __restric
   for (int c = 0; c < 3; ++c) {
     const size_t idx = vo + (size_t)(3 * s + c) * 64 + l64;
     float4 o;
-    o.x = cf0.x * (ca.x * acc0[c].x + ca.y * ctr[c].x);
-    o.y = cf0.x * (ca.x * acc0[c].y + ca.y * ctr[c].y);
-    o.z = cf1.x * (ca.x * acc1[c].x + ca.y * ctr[c].z);
-    o.w = cf1.x * (ca.x * acc1[c].y + ca.y * ctr[c].w);
+    o.x = cf0.x * (ca.x * acc0[c].x + ca.y * ctr[c].x) + cf0.w * ctr[c].x;
+    o.y = cf0.x * (ca.x * acc0[c].y + ca.y * ctr[c].y) + cf0.w * ctr[c].y;
+    o.z = cf1.x * (ca.x * acc1[c].x + ca.y * ctr[c].z) + cf1.w * ctr[c].z;
+    o.w = cf1.x * (ca.x * acc1[c].y + ca.y * ctr[c].w) + cf1.w * ctr[c].w;
     if (HAS_R) {
       const float4 rv = ld_nt4(r + idx);
       o.x += cf0.y * rv.x;
@@ -387,8 +388,8 @@ __global__ __launch_bounds__(256) void k_filter_step_bf16(const float2* __restri
 #pragma unroll
     for (int j = 0; j < VPL; ++j) {
       const float2 xc = bf2_unpack(ctr[c][j]);
-      float2 val = make_float2(cf[j].x * (ca.x * acc[c][j].x + ca.y * xc.x),
-                               cf[j].x * (ca.x * acc[c][j].y + ca.y * xc.y));
+      float2 val = make_float2(cf[j].x * (ca.x * acc[c][j].x + ca.y * xc.x) + cf[j].w * xc.x,
+                               cf[j].x * (ca.x * acc[c][j].y + ca.y * xc.y) + cf[j].w * xc.y);
       if (HAS_R) {
         const float2 rr = bf2_unpack(rv[j]);
         val.x += cf[j].y * rr.x;

@@ -121,12 +121,15 @@ def test_filter_block_widths(lp, gpu, storage, width, monkeypatch):
     b.close()
 
 
-@pytest.mark.parametrize("passes", [1, 2, 3, 4])
-def test_filter_passes(lp, gpu, passes):
+@pytest.mark.parametrize("form,passes", [(1, 1), (1, 2), (1, 3), (1, 4), (0, 1), (0, 4)])
+def test_filter_passes(lp, gpu, form, passes, monkeypatch):
     """T_m(B)^P per cycle (correction pass + P-1 plain passes, each adding
-    gamma^(p-1) f_m in its last step) gives the dense oracle's spectrum."""
+    gamma^(p-1) f_m in its last step) gives the dense oracle's spectrum, in
+    the product form over the roots of T_m (form 1, the default) and in the
+    three-term recurrence (form 0)."""
     L, nts, nev, deg, tol = 8, 2, 40, 8, 1e-10
     data = smeared(L, 31, 0.4, nts)
+    monkeypatch.setenv("LAPH_FILTER_FORM", str(form))
     b = lp.Batch(L, deg, 12.0, 0.0, nts, nev, nsub=64, tol=tol, seed=2, device=gpu, filter_passes=passes)
     for t, (ts, _) in enumerate(data):
         b.set_timeslice(t, ts, 0.62, 0.62, 3)
