@@ -43,7 +43,7 @@ PEAK_HBM = 8.0e12
 # isolated replays of --profile-filter, each scaled by the same counter on a
 # 1 GiB device copy (python bench.py --summarize-pmc ...); None = not measured
 # for this build.
-TRAFFIC_BYTES_PER_LAUNCH = 29074033011  # profiles/filter_step_32x32x32x64_k256_pmc.md
+TRAFFIC_BYTES_PER_LAUNCH = 23490735081  # profiles/filter_step_32x32x32x64_k256_pmc.md
 
 
 def main():
@@ -139,11 +139,11 @@ def main():
                 "in_solve_GBps": tot["fbytes"] / tot["filter"] / 1e9 if tot["filter"] > 0 else None}
 
     if not args.no_stream:
-        # what a plain streaming kernel reaches for the filter's access mix
-        # (3 reads + 1 write, non-temporal) on this device: the practical ceiling
+        # what a plain streaming kernel reaches for the timed step's access mix
+        # (2 reads + 1 write, non-temporal) on this device: the practical ceiling
         sc = stream_ceiling()
-        roofline["stream_3r1w_GBps"] = sc.get("3r1w_nt_GBps")
-        roofline["frac_of_stream"] = achieved / 1e9 / sc["3r1w_nt_GBps"] if sc.get("3r1w_nt_GBps") else None
+        roofline["stream_2r1w_GBps"] = sc.get("2r1w_nt_GBps")
+        roofline["frac_of_stream"] = achieved / 1e9 / sc["2r1w_nt_GBps"] if sc.get("2r1w_nt_GBps") else None
 
     # plain FP64 drop-in on a few timeslices of the same field
     lamC = 1.35 * abs(lam_last)

@@ -823,20 +823,21 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
       const int cols = K - (nl / 64) * 64;
       const int nblk = (cols + b->bw - 1) / b->bw;  // blocks the kernel visits (links once each)
       ln->sv32 += (long long)(b->deg - 1) * cols;
-      // algorithmic bytes: correction step 1 reads r and writes f_2 (2 vector
-      // touches), step 2 adds r (3), later steps add f_{k-1} (4); plain-pass
-      // step 1 in/out (2), later steps in/old/out (3), the last one also f (4).
+      // algorithmic bytes (vector touches per step): product form, correction
+      // step 1 reads r and writes e_2 (2), later steps read e_k and r and
+      // write (3); plain-pass steps read and write (2), the last one also
+      // reads f (3).  Three-term form: later correction steps also read
+      // e_{k-1} (4), plain-pass steps after the first the previous vector.
       // A vector touch is 3 colours x esz bytes per site; links 27 x 8 B per
       // site per vector block
       for (int s = 1; s < b->deg; ++s) {
-        // product form: read in and r, write out; three-term: also the previous vector
-      const int touches = s == 1 ? 2 : (s == 2 || b->prod ? 3 : 4);
+        const int touches = s == 1 ? 2 : (s == 2 || b->prod ? 3 : 4);
         ln->fbytes += (double)b->V3 * ((double)cols * touches * 3.0 * b->esz + nblk * 216.0);
       }
       for (int p = 2; p <= passes; ++p) {
         ln->sv32 += (long long)b->deg * cols;
         for (int s = 1; s <= b->deg; ++s) {
-          const int touches = s == 1 ? 2 : (s == b->deg ? 4 : 3);
+          const int touches = (s == 1 || b->prod ? 2 : 3) + (s == b->deg ? 1 : 0);
           ln->fbytes += (double)b->V3 * ((double)cols * touches * 3.0 * b->esz + nblk * 216.0);
         }
       }
@@ -1150,7 +1151,7 @@ int laph_batch_time_filter(laph_batch* b, int reps, double* ms_per_launch, doubl
     const int steps = b->deg - 1;
     double bytes = 0;
     for (int s = 1; s < b->deg; ++s) {
-      const int touches = s == 1 ? 2 : (s == 2 ? 3 : 4);
+      const int touches = s == 1 ? 2 : (s == 2 || b->prod ? 3 : 4);
       bytes += (double)nt * b->V3 * ((double)K * touches * 3.0 * b->esz + (K / b->bw) * 216.0);
     }
     *ms_per_launch = ms / ((double)reps * steps);
