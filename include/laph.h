@@ -171,13 +171,15 @@ typedef struct laph_batch laph_batch;
 
 /* Environment controls of a batch, read by laph_batch_create (for tuning and
  * tests; the defaults are the measured best):
- *   LAPH_LANES=1..4         host lanes (threads, streams) the timeslices are split over (2)
+ *   LAPH_LANES=1..4         host lanes (threads, streams) the timeslices are split over (1)
  *   LAPH_BW=64|128|256      vectors per filter block (else timed on the first solve)
  *   LAPH_STRIP=w            sweep strip width of the stencils, w divides L (else timed)
  *   LAPH_FILTER_FORM=0|1    T_m(B) applied as a product over its Leja-ordered roots (1), or by the
  *                           three-term recurrence, which also reads the previous vector (0)
  *   LAPH_WARM=n             single-pass cycles before T_m(B)^P cycles (2)
  *   LAPH_QR_CYCLES=n        cycles (and start block) orthonormalised by explicit CholQR2 (0)
+ *   LAPH_START_RR=1         Rayleigh-Ritz on the Gaussian start block before the first cycle (off: the
+ *                           first cycle filters the block as it is)
  *   LAPH_SHIFTED_CHOLQR=1   every CholQR pass starts with a shifted one
  *   LAPH_EIG=0|1|2          Rayleigh-Ritz eigensolver: zheevd, zheevdj, zheevj (0)
  *   LAPH_TRACE=1            per-cycle trace of the first timeslice on stderr

@@ -112,7 +112,7 @@ static int batch_alloc(laph_batch* b) {
   } else if (b->K % cand[1] != 0) {
     b->bw = cand[0];
   }
-  int nlanes = 2;
+  int nlanes = 1;
   if (const char* e = std::getenv("LAPH_LANES")) nlanes = std::max(1, std::min(4, std::atoi(e)));
   nlanes = std::min(nlanes, b->nts);
   b->lanes.resize(nlanes);
@@ -420,6 +420,21 @@ static int orthonormalise_and_rr(const View& v, Act q, bool explicit_qr) {
     if (!failed) return LAPH_OK;
   }
   return set_error(LAPH_ERR_NOCONV, "Rayleigh-Ritz: Gram matrix not positive definite after CholQR2");
+}
+
+// Start of a solve from the Gaussian block without a Rayleigh-Ritz step: a
+// rotation of the block does not change its span and the filter is linear,
+// so the first (single-pass) filter cycle can take the columns as they are,
+// each with theta = 1, the window edge (T_m(1) = 1, gamma = 1: the filtered
+// block is X + e).  Needs D X for the residuals; the window lower end sits at
+// its cap, 0.75 lambda_l, which a random block's largest Ritz value exceeds.
+static int start_from_block(const View& v) {
+  laph_batch* b = v.b;
+  LAPH_HIP(launch_lap_block64(b->U64 + v.lo(), v.X(), v.AX(), b->p.L, b->K, v.ln->nt, b->lock_blk + v.ln->t0,
+                              b->strip, v.ln->st));
+  v.ln->sv64 += (long long)v.ln->nt * b->K;
+  std::fill(b->h_mu.begin() + v.nk(), b->h_mu.begin() + v.nk() + (size_t)v.ln->nt * b->K, 0.75 * b->p.lambda_l);
+  return LAPH_OK;
 }
 
 // Window per timeslice from the Ritz values, residuals of the active block,
@@ -807,7 +822,10 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
   }
   // start block (network proposal or seeded Gaussian), CholQR2, first Rayleigh-Ritz
   if (!warm) LAPH_HIP(launch_random_block(v.X(), (size_t)nt * b->per_ts, b->o.seed + 0x1000193ull * ln->t0, ln->st));
-  LAPH_TRY(orthonormalise_and_rr(v, Act{0, K}, b->qr_cycles > 0));
+  if (warm || b->start_rr || b->qr_cycles > 0 || b->warm_cycles == 0)
+    LAPH_TRY(orthonormalise_and_rr(v, Act{0, K}, b->qr_cycles > 0));
+  else
+    LAPH_TRY(start_from_block(v));
   std::vector<double> lc(nt);
   int active = 0;
   LAPH_TRY(residuals_and_window(v, lc, &active));
@@ -1058,6 +1076,7 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     if (const char* e = std::getenv("LAPH_QR_CYCLES")) b->qr_cycles = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("LAPH_TEST_POISON")) b->poison_ts = std::atoi(e);
     if (const char* e = std::getenv("LAPH_FILTER_FORM")) b->prod = std::atoi(e) != 0;
+    if (const char* e = std::getenv("LAPH_START_RR")) b->start_rr = std::atoi(e) != 0;
     b->roots = laph::leja_roots(p->deg);
     b->strip = strip > 0 ? strip : p->L;
     b->strip_fixed = strip > 0;

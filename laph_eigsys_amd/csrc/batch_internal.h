@@ -105,6 +105,7 @@ struct laph_batch {
   bool prod = true;          // product-form filter over Leja-ordered roots of T_m; LAPH_FILTER_FORM=0:
                              // three-term recurrence
   std::vector<double> roots; // roots of T_deg in the order the product form applies them
+  bool start_rr = false;     // Rayleigh-Ritz on the Gaussian start block before the first cycle (LAPH_START_RR=1)
   bool force_shift = false;  // every CholQR pass starts with a shifted one (LAPH_SHIFTED_CHOLQR)
 };
 

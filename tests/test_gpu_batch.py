@@ -157,18 +157,40 @@ def test_shifted_cholqr_path(lp, gpu, monkeypatch):
     b.close()
 
 
-@pytest.mark.parametrize("L,nts,nev,nsub", [(5, 3, 16, 64), (7, 1, 30, 64), (9, 2, 60, 64)])
-def test_batch_edge_shapes(lp, gpu, L, nts, nev, nsub):
+@pytest.mark.parametrize("L,nts,nev,nsub,lanes", [(5, 3, 16, 64, 2), (7, 1, 30, 64, 1), (9, 2, 60, 64, 2),
+                                                   (6, 4, 20, 64, 3)])
+def test_batch_edge_shapes(lp, gpu, L, nts, nev, nsub, lanes, monkeypatch):
     """Odd L and L not divisible by 4 (plain sweep order, partial last
-    site-block), a single timeslice (one lane), three timeslices over two
-    lanes, and nev close to the search-space width."""
+    site-block), a single timeslice, timeslices split over two or three
+    lanes (host threads and streams, LAPH_LANES), and nev close to the
+    search-space width."""
     tol = 1e-10
     data = smeared(L, 4000 + L, 0.4, nts)
+    monkeypatch.setenv("LAPH_LANES", str(lanes))
     b = lp.Batch(L, 10, 12.0, 0.0, nts, nev, nsub=nsub, tol=tol, seed=3, device=gpu)
     for t, (ts, _) in enumerate(data):
         b.set_timeslice(t, ts, 0.62, 0.62, 3)
     st = b.solve()
-    assert st.converged == nts and st.filter_strip == L
+    assert st.converged == nts and st.filter_strip in ((L,) if L % 4 else (L, 4))
+    for t, (_, U) in enumerate(data):
+        ev, V, ph, res = b.get(t)
+        check_pairs(ev, V, res, N.dense_D(U.reshape(L, L, L, 3, 3, 3)), nev, tol)
+    b.close()
+
+
+@pytest.mark.parametrize("start_rr", [0, 1])
+def test_start_block(lp, gpu, start_rr, monkeypatch):
+    """The first filter cycle straight from the Gaussian start block (every
+    column at the window edge, theta = 1) and after a Rayleigh-Ritz step on it
+    (LAPH_START_RR=1) span the same subspace: both give the dense oracle's
+    spectrum."""
+    L, nts, nev, deg, tol = 8, 2, 40, 10, 1e-10
+    data = smeared(L, 77, 0.4, nts)
+    monkeypatch.setenv("LAPH_START_RR", str(start_rr))
+    b = lp.Batch(L, deg, 12.0, 0.0, nts, nev, nsub=64, tol=tol, seed=9, device=gpu)
+    for t, (ts, _) in enumerate(data):
+        b.set_timeslice(t, ts, 0.62, 0.62, 3)
+    assert b.solve().converged == nts
     for t, (_, U) in enumerate(data):
         ev, V, ph, res = b.get(t)
         check_pairs(ev, V, res, N.dense_D(U.reshape(L, L, L, 3, 3, 3)), nev, tol)
