@@ -181,6 +181,8 @@ typedef struct laph_batch laph_batch;
  *   LAPH_START_RR=1         Rayleigh-Ritz on the Gaussian start block before the first cycle (off: the
  *                           first cycle filters the block as it is)
  *   LAPH_SHIFTED_CHOLQR=1   every CholQR pass starts with a shifted one
+ *   LAPH_TRSM=1             Rayleigh-Ritz through triangular solves with the Gram's Cholesky factor
+ *                           (else its explicit inverse and GEMMs)
  *   LAPH_EIG=0|1|2          Rayleigh-Ritz eigensolver: zheevd, zheevdj, zheevj (0)
  *   LAPH_TRACE=1            per-cycle trace of the first timeslice on stderr
  *   LAPH_TEST_POISON=t      (tests) NaN written into one row of timeslice t's unlocked columns in

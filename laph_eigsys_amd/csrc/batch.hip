@@ -80,6 +80,8 @@ static int batch_alloc(laph_batch* b) {
   LAPH_HIP(hipMalloc(&b->G, kk * sizeof(double2)));
   LAPH_HIP(hipMalloc(&b->G2, kk * sizeof(double2)));
   LAPH_HIP(hipMalloc(&b->P2, kk * sizeof(double2)));
+  LAPH_HIP(hipMalloc(&b->G3, kk * sizeof(double2)));
+  LAPH_HIP(hipMemset(b->G3, 0, kk * sizeof(double2)));
   const size_t nk = (size_t)b->nts * b->K;
   LAPH_HIP(hipMalloc(&b->mu, nk * sizeof(double)));
   LAPH_HIP(hipMalloc(&b->E, nk * sizeof(double)));
@@ -151,7 +153,7 @@ static void batch_free(laph_batch* b) {
   for (void* q : {(void*)b->wV, (void*)b->wph, (void*)b->wG})
     if (q) (void)hipFree(q);
   void* bufs[] = {b->U64, b->U32, b->dec,  b->Unew, b->B[0], b->B[1],   b->B[2], b->R32,     b->P[0],
-                  b->P[1], b->P[2], b->G,  b->G2,   b->P2,   b->mu,     b->E,    b->info,    b->nrm2,
+                  b->P[1], b->P[2], b->G,  b->G2,   b->P2,   b->G3,   b->mu,     b->E,    b->info,    b->nrm2,
                   b->lam,  b->cts,  b->gam, b->coef, b->cab, b->first_blk, b->lock_blk, b->cn, b->coef2, b->jsweeps};
   for (void* q : bufs)
     if (q) (void)hipFree(q);
@@ -327,6 +329,15 @@ __global__ void k_scale_rows(double2* __restrict__ Z, const double* __restrict__
   Z[e].y *= f;
 }
 
+// Clears the strictly lower triangle of the a x a leading block (lda K) of
+// every timeslice's matrix: the triangular inverse enters full GEMMs.
+__global__ void k_zero_lower(double2* __restrict__ W, int a, int K) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a * a) return;
+  const int i = idx % a, j = idx / a;
+  if (i > j) W[(size_t)blockIdx.y * K * K + (size_t)j * K + i] = make_double2(0.0, 0.0);
+}
+
 static int rayleigh_ritz(const View& v, Act q, bool orthonormal, bool* chol_failed) {
   laph_batch* b = v.b;
   const int K = b->K, nt = v.ln->nt;
@@ -336,6 +347,8 @@ static int rayleigh_ritz(const View& v, Act q, bool orthonormal, bool* chol_fail
   double2* T = v.T();
   double2* G = b->G + v.kk();
   double2* G2 = b->G2 + v.kk();
+  double2* W = b->G3 + v.kk();  // U^{-1}
+  double2* T1 = b->P2 + v.kk();  // free during Rayleigh-Ritz
   rocblas_int* info = b->info + v.ln->t0;
   const rocblas_stride kk = (rocblas_stride)K * K;
   rocblas_handle rb = v.ln->rb;
@@ -353,11 +366,25 @@ static int rayleigh_ritz(const View& v, Act q, bool orthonormal, bool* chol_fail
       *chol_failed = true;
       return LAPH_OK;
     }
-    LAPH_RB(rocblas_ztrsm_strided_batched(rb, rocblas_side_left, rocblas_fill_upper,
-                                          rocblas_operation_conjugate_transpose, rocblas_diagonal_non_unit, q.a, q.a,
-                                          &one, zc(G2), K, kk, zc(G), K, kk, nt));
-    LAPH_RB(rocblas_ztrsm_strided_batched(rb, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
-                                          rocblas_diagonal_non_unit, q.a, q.a, &one, zc(G2), K, kk, zc(G), K, kk, nt));
+    if (b->use_trsm) {
+      LAPH_RB(rocblas_ztrsm_strided_batched(rb, rocblas_side_left, rocblas_fill_upper,
+                                            rocblas_operation_conjugate_transpose, rocblas_diagonal_non_unit, q.a,
+                                            q.a, &one, zc(G2), K, kk, zc(G), K, kk, nt));
+      LAPH_RB(rocblas_ztrsm_strided_batched(rb, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                                            rocblas_diagonal_non_unit, q.a, q.a, &one, zc(G2), K, kk, zc(G), K, kk,
+                                            nt));
+    } else {
+      // W = U^{-1} explicitly (a few dispatches), then U^{-H} Mt U^{-1} as two
+      // small GEMMs: the triangular solves cost hundreds of tiny dispatches
+      LAPH_RB(rocblas_ztrtri_strided_batched(rb, rocblas_fill_upper, rocblas_diagonal_non_unit, q.a, zc(G2), K, kk,
+                                             zc(W), K, kk, nt));
+      k_zero_lower<<<ga, 256, 0, v.ln->st>>>(W, q.a, K);
+      LAPH_HIP(hipGetLastError());
+      LAPH_RB(rocblas_zgemm_strided_batched(rb, rocblas_operation_none, rocblas_operation_none, q.a, q.a, q.a, &one,
+                                            zc(G), K, kk, zc(W), K, kk, &zero, zc(T1), K, kk, nt));
+      LAPH_RB(rocblas_zgemm_strided_batched(rb, rocblas_operation_conjugate_transpose, rocblas_operation_none, q.a,
+                                            q.a, q.a, &one, zc(W), K, kk, zc(T1), K, kk, &zero, zc(G), K, kk, nt));
+    }
   }
   if (b->eig_mode == 1) {
     LAPH_RB(rocsolver_zheevdj_strided_batched(rb, rocblas_evect_original, rocblas_fill_upper, q.a, zc(G), K, kk,
@@ -371,14 +398,22 @@ static int rayleigh_ritz(const View& v, Act q, bool orthonormal, bool* chol_fail
                                              b->mu + v.nk() + q.l, K, b->E + v.nk(), K, info, nt));
   }
   LAPH_TRY(dense_sync_info(v, "Rayleigh-Ritz (zheevd)"));
+  double2* Z = G;  // rotation of the active block
   if (!orthonormal) {
-    LAPH_RB(rocblas_ztrsm_strided_batched(rb, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
-                                          rocblas_diagonal_non_unit, q.a, q.a, &one, zc(G2), K, kk, zc(G), K, kk, nt));
-    k_scale_rows<<<ga, 256, 0, v.ln->st>>>(G, sc, q.a, K);
+    if (b->use_trsm) {
+      LAPH_RB(rocblas_ztrsm_strided_batched(rb, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                                            rocblas_diagonal_non_unit, q.a, q.a, &one, zc(G2), K, kk, zc(G), K, kk,
+                                            nt));
+    } else {
+      LAPH_RB(rocblas_zgemm_strided_batched(rb, rocblas_operation_none, rocblas_operation_none, q.a, q.a, q.a, &one,
+                                            zc(W), K, kk, zc(G), K, kk, &zero, zc(T1), K, kk, nt));
+      Z = T1;
+    }
+    k_scale_rows<<<ga, 256, 0, v.ln->st>>>(Z, sc, q.a, K);
     LAPH_HIP(hipGetLastError());
   }
   LAPH_RB(rocblas_zgemm_strided_batched(rb, rocblas_operation_conjugate_transpose, rocblas_operation_none, q.a, b->n,
-                                        q.a, &one, zc(G), K, kk, zc(Y + q.l), K, (rocblas_stride)b->per_ts, &zero,
+                                        q.a, &one, zc(Z), K, kk, zc(Y + q.l), K, (rocblas_stride)b->per_ts, &zero,
                                         zc(T + q.l), K, (rocblas_stride)b->per_ts, nt));
   LAPH_HIP(launch_lap_block64(b->U64 + v.lo(), T, AY, b->p.L, K, nt, b->lock_blk + v.ln->t0, b->strip, v.ln->st));
   // the hard-locked columns of X live in the old X buffer: carry them over
@@ -1077,6 +1112,7 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     if (const char* e = std::getenv("LAPH_TEST_POISON")) b->poison_ts = std::atoi(e);
     if (const char* e = std::getenv("LAPH_FILTER_FORM")) b->prod = std::atoi(e) != 0;
     if (const char* e = std::getenv("LAPH_START_RR")) b->start_rr = std::atoi(e) != 0;
+    if (const char* e = std::getenv("LAPH_TRSM")) b->use_trsm = std::atoi(e) != 0;
     b->roots = laph::leja_roots(p->deg);
     b->strip = strip > 0 ? strip : p->L;
     b->strip_fixed = strip > 0;

@@ -61,7 +61,8 @@ struct laph_batch {
   // small per-timeslice data
   double2* G = nullptr;      // [nts][K*K] Rayleigh-Ritz / CholQR matrix
   double2* G2 = nullptr;     // [nts][K*K] Gram of the active block
-  double2* P2 = nullptr;     // [nts][K*K] projections on the locked block
+  double2* P2 = nullptr;     // [nts][K*K] projections on the locked block; Rayleigh-Ritz scratch
+  double2* G3 = nullptr;     // [nts][K*K] inverse of the Gram's Cholesky factor
   double* mu = nullptr;      // [nts][K]
   double* E = nullptr;       // [nts][K]
   rocblas_int* info = nullptr;
@@ -105,6 +106,7 @@ struct laph_batch {
   bool prod = true;          // product-form filter over Leja-ordered roots of T_m; LAPH_FILTER_FORM=0:
                              // three-term recurrence
   std::vector<double> roots; // roots of T_deg in the order the product form applies them
+  bool use_trsm = false;     // triangular solves instead of the explicit inverse in Rayleigh-Ritz (LAPH_TRSM=1)
   bool start_rr = false;     // Rayleigh-Ritz on the Gaussian start block before the first cycle (LAPH_START_RR=1)
   bool force_shift = false;  // every CholQR pass starts with a shifted one (LAPH_SHIFTED_CHOLQR)
 };

@@ -178,15 +178,17 @@ def test_batch_edge_shapes(lp, gpu, L, nts, nev, nsub, lanes, monkeypatch):
     b.close()
 
 
-@pytest.mark.parametrize("start_rr", [0, 1])
-def test_start_block(lp, gpu, start_rr, monkeypatch):
+@pytest.mark.parametrize("start_rr,trsm", [(0, 0), (1, 0), (0, 1)])
+def test_start_block(lp, gpu, start_rr, trsm, monkeypatch):
     """The first filter cycle straight from the Gaussian start block (every
     column at the window edge, theta = 1) and after a Rayleigh-Ritz step on it
     (LAPH_START_RR=1) span the same subspace: both give the dense oracle's
-    spectrum."""
+    spectrum; so does Rayleigh-Ritz through triangular solves (LAPH_TRSM=1)
+    instead of the explicit inverse of the Gram's Cholesky factor."""
     L, nts, nev, deg, tol = 8, 2, 40, 10, 1e-10
     data = smeared(L, 77, 0.4, nts)
     monkeypatch.setenv("LAPH_START_RR", str(start_rr))
+    monkeypatch.setenv("LAPH_TRSM", str(trsm))
     b = lp.Batch(L, deg, 12.0, 0.0, nts, nev, nsub=64, tol=tol, seed=9, device=gpu)
     for t, (ts, _) in enumerate(data):
         b.set_timeslice(t, ts, 0.62, 0.62, 3)
