@@ -38,12 +38,12 @@ L, LT, EPS, NEV, DEG, TOL, NSUB = 32, 64, 0.3, 220, 16, 1e-10, 256
 SEED = 1227
 LAMBDA_L = 12.0
 PEAK_HBM = 8.0e12
-# HBM bytes per filter-step launch (all columns active, 32 timeslices of one
-# lane, 128-vector blocks, y-strips of 4) from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the
-# isolated replays of --profile-filter, each scaled by the same counter on a
-# 1 GiB device copy (python bench.py --summarize-pmc ...); None = not measured
-# for this build.
-TRAFFIC_BYTES_PER_LAUNCH = 23490735081  # profiles/filter_step_32x32x32x64_k256_pmc.md
+# HBM bytes per filter-step launch (all columns active, the 64 timeslices of
+# the default single lane, 128-vector blocks, y-strips of 4) from rocprofv3
+# --pmc FETCH_SIZE / WRITE_SIZE on the isolated replays of --profile-filter,
+# each scaled by the same counter on a 1 GiB device copy (python bench.py
+# --summarize-pmc ...); None = not measured for this build.
+TRAFFIC_BYTES_PER_LAUNCH = 46981408061  # profiles/filter_step_32x32x32x64_k256_pmc.md
 
 
 def main():
@@ -208,7 +208,7 @@ def profile_filter(lp, args):
     hip.hipFree(src)
     hip.hipFree(dst)
     print(json.dumps({"filter_ms_per_launch": ms, "filter_bytes_per_launch": by, "filter_GBps": by / ms / 1e6,
-                      "lane_timeslices": nts - nts // 2, "copy_bytes": nbytes,
+                      "lane_timeslices": nts // int(os.environ.get("LAPH_LANES", "1")), "copy_bytes": nbytes,
                       "isolated_filter_dispatches": (args.filter_reps + 1) * (DEG - 1)}))
 
 
@@ -240,6 +240,50 @@ def summarize_trace(db, last):
     if last:
         fl = [(t1 - t0) / 1e3 for name, t0, t1 in rows if _filter_kernel(name)][-last:]
         print(f"isolated filter replays: last {len(fl)} filter dispatches, mean {sum(fl) / len(fl):.2f} us")
+    _solve_phases(rows)
+
+
+def _phase(name):
+    if _filter_kernel(name):
+        return "filter (FP32 stencil steps)"
+    if name.startswith("Cijk") and ("MT64x64" in name or "MT128x64" in name):
+        return "large GEMMs (Gram, Rayleigh quotient, rotation, projection)"
+    if "laph" in name:
+        return "other library kernels (FP64 stencil, residual, combine, ...)"
+    if "rocsolver" in name or "rocblas" in name or name.startswith("Cijk"):
+        return "rocSOLVER / small rocBLAS (Cholesky, inverse, eigensolver)"
+    return "runtime copies and fills"
+
+
+def _solve_phases(rows):
+    """Wall-clock split of the last batched solve in the trace (from its
+    Gaussian start block to its last residual): per phase, the time from the
+    first to the last dispatch of each run of consecutive same-phase
+    dispatches (kernels plus the gaps between them) and the kernel time."""
+    starts = [t0 for name, t0, t1 in rows if "k_random_block" in name]
+    ends = [t1 for name, t0, t1 in rows if "k_residual" in name]
+    if not starts or not ends:
+        return
+    sel = [r for r in rows if starts[-1] <= r[1] <= ends[-1]]
+    runs = []
+    for name, t0, t1 in sel:
+        ph = _phase(name)
+        if runs and runs[-1][0] == ph:
+            runs[-1][2] = max(runs[-1][2], t1)
+            runs[-1][3] += t1 - t0
+            runs[-1][4] += 1
+        else:
+            runs.append([ph, t0, t1, t1 - t0, 1])
+    agg = {}
+    for ph, t0, t1, busy, cnt in runs:
+        a = agg.setdefault(ph, [0.0, 0.0, 0])
+        a[0] += (t1 - t0) / 1e6
+        a[1] += busy / 1e6
+        a[2] += cnt
+    span = (sel[-1][2] - sel[0][1]) / 1e6
+    print(f"last solve: {span:.1f} ms from its start block to its last residual")
+    for ph, (wall, busy, cnt) in sorted(agg.items(), key=lambda kv: -kv[1][0]):
+        print(f"  {ph:62s} {wall:8.1f} ms wall {busy:8.1f} ms kernels {cnt:6d} dispatches")
 
 
 def summarize_pmc(fetch_csv, write_csv, last):
