@@ -43,7 +43,7 @@ PEAK_HBM = 8.0e12
 # --pmc FETCH_SIZE / WRITE_SIZE on the isolated replays of --profile-filter,
 # each scaled by the same counter on a 1 GiB device copy (python bench.py
 # --summarize-pmc ...); None = not measured for this build.
-TRAFFIC_BYTES_PER_LAUNCH = 46981408061  # profiles/filter_step_32x32x32x64_k256_pmc.md
+TRAFFIC_BYTES_PER_LAUNCH = None  # re-measured for the correction pass without r reads
 
 
 def main():
@@ -140,10 +140,10 @@ def main():
 
     if not args.no_stream:
         # what a plain streaming kernel reaches for the timed step's access mix
-        # (2 reads + 1 write, non-temporal) on this device: the practical ceiling
+        # (1 read + 1 write, non-temporal) on this device: the practical ceiling
         sc = stream_ceiling()
-        roofline["stream_2r1w_GBps"] = sc.get("2r1w_nt_GBps")
-        roofline["frac_of_stream"] = achieved / 1e9 / sc["2r1w_nt_GBps"] if sc.get("2r1w_nt_GBps") else None
+        roofline["stream_1r1w_GBps"] = sc.get("1r1w_nt_GBps")
+        roofline["frac_of_stream"] = achieved / 1e9 / sc["1r1w_nt_GBps"] if sc.get("1r1w_nt_GBps") else None
 
     # plain FP64 drop-in on a few timeslices of the same field
     lamC = 1.35 * abs(lam_last)

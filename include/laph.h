@@ -174,8 +174,10 @@ typedef struct laph_batch laph_batch;
  *   LAPH_LANES=1..4         host lanes (threads, streams) the timeslices are split over (1)
  *   LAPH_BW=64|128|256      vectors per filter block (else timed on the first solve)
  *   LAPH_STRIP=w            sweep strip width of the stencils, w divides L (else timed)
- *   LAPH_FILTER_FORM=0|1    T_m(B) applied as a product over its Leja-ordered roots (1), or by the
- *                           three-term recurrence, which also reads the previous vector (0)
+ *   LAPH_FILTER_FORM=0|1|2  T_m(B) applied as a product over its Leja-ordered roots, the correction
+ *                           pass as a product over the roots of (T_m(y) - T_m(theta)) / (y - theta)
+ *                           applied to r (2); the same with the correction pass reading r in every
+ *                           step (1); the three-term recurrence, which also reads the previous vector (0)
  *   LAPH_WARM=n             single-pass cycles before T_m(B)^P cycles (2)
  *   LAPH_QR_CYCLES=n        cycles (and start block) orthonormalised by explicit CholQR2 (0)
  *   LAPH_START_RR=1         Rayleigh-Ritz on the Gaussian start block before the first cycle (off: the

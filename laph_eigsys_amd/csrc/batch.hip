@@ -39,6 +39,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <functional>
 #include <string>
@@ -581,6 +582,52 @@ static void product_coefficients(const laph_batch* b, double th, int m, float4* 
   *g = std::pow(gam, passes);
 }
 
+// Order in which the correction pass applies the roots x_k = cos(phi +
+// 2 pi k / m), k = 1 .. m-1, of q_theta (below): Leja order of those points
+// for a theta just above the window edge (phi = 0.1 i), where the pairs k,
+// m - k nearly coincide; one order serves every column.
+static std::vector<int> correction_root_order(int m) {
+  std::vector<std::complex<double>> z;
+  for (int k = 1; k < m; ++k) z.push_back(std::cos(std::complex<double>(2.0 * M_PI * k / m, 0.1)));
+  std::vector<int> order;
+  std::vector<bool> used(z.size(), false);
+  for (size_t n = 0; n < z.size(); ++n) {
+    int best = -1;
+    double bv = -INFINITY;
+    for (size_t i = 0; i < z.size(); ++i) {
+      if (used[i]) continue;
+      double lp = 0.0;
+      for (int j : order) lp += std::log(std::abs(z[i] - z[j - 1]));
+      if (n == 0) lp = std::abs(z[i]);
+      if (lp > bv) bv = lp, best = (int)i;
+    }
+    used[best] = true;
+    order.push_back(best + 1);
+  }
+  return order;
+}
+
+// Correction pass as a plain product (the default): with r = (B - theta) x,
+// T_m(B) x - T_m(theta) x = q(B) r, q(y) = (T_m(y) - T_m(theta)) / (y - theta)
+// = 2^(m-1) prod_{k=1}^{m-1} (y - x_k), x_k = cos(phi + 2 pi k / m), theta =
+// cos phi (complex phi for theta > 1: complex shifts).  Step st makes
+// h_st = q_st(B) r / sigma_st from h_{st-1} (h_0 = r), sigma_st = max(|q_st(theta)|, 1)
+// and sigma_{m-1} = sm, so the pass ends with e_m / sm without reading r again.
+static void correction_q_coefficients(const laph_batch* b, double th, int m, double sm, float4* co, size_t stride) {
+  const std::complex<double> phi = std::acos(std::complex<double>(th, 0.0));
+  std::complex<double> qk(1.0, 0.0);
+  double sig = 1.0;
+  for (int st = 1; st < m; ++st) {
+    const std::complex<double> x = std::cos(phi + 2.0 * M_PI * b->qorder[st - 1] / m);
+    qk *= 2.0 * (th - x);
+    const double sn = st == m - 1 ? sm : std::max(std::abs(qk), 1.0);
+    const double a1 = 2.0 * sig / sn;
+    const std::complex<double> a3 = -a1 * x;
+    co[(size_t)(st - 1) * stride] = make_float4((float)a1, 0.f, (float)a3.imag(), (float)a3.real());
+    sig = sn;
+  }
+}
+
 static int upload_coefficients(const View& v, const std::vector<double>& lc, int passes) {
   laph_batch* b = v.b;
   laph_lane* ln = v.ln;
@@ -604,6 +651,7 @@ static int upload_coefficients(const View& v, const std::vector<double>& lc, int
       if (b->prod) {
         product_coefficients(b, th, m, &co[(size_t)t * K + j], &co2[(size_t)t * K + j], (size_t)nt * K,
                              &ga[(size_t)t * K + j], passes, tau.data(), s.data());
+        if (b->qform) correction_q_coefficients(b, th, m, s[m], &co[(size_t)t * K + j], (size_t)nt * K);
         continue;
       }
       tau[0] = 1.0;
@@ -662,7 +710,8 @@ static int enqueue_filter(const View& v) {
     const void* in = st == 1 ? R : fb(b, b->P[(st - 2) % nb], vo);
     const void* old = st >= 3 && !b->prod ? fb(b, b->P[(st - 3) % 3], vo) : nullptr;
     void* out = fb(b, b->P[(st - 1) % nb], vo);
-    LAPH_HIP(launch_filter_step(b->U32 + v.lo(), in, R, old, out, b->coef + (st - 1) * step + v.nk(),
+    LAPH_HIP(launch_filter_step(b->U32 + v.lo(), in, b->qform ? nullptr : R, old, out,
+                                b->coef + (st - 1) * step + v.nk(),
                                 b->cab + v.ln->t0, b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, b->strip, v.ln->st));
   }
   return LAPH_OK;
@@ -884,7 +933,7 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
       // A vector touch is 3 colours x esz bytes per site; links 27 x 8 B per
       // site per vector block
       for (int s = 1; s < b->deg; ++s) {
-        const int touches = s == 1 ? 2 : (s == 2 || b->prod ? 3 : 4);
+        const int touches = s == 1 || b->qform ? 2 : (s == 2 || b->prod ? 3 : 4);
         ln->fbytes += (double)b->V3 * ((double)cols * touches * 3.0 * b->esz + nblk * 216.0);
       }
       for (int p = 2; p <= passes; ++p) {
@@ -1110,7 +1159,11 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     b->force_shift = std::getenv("LAPH_SHIFTED_CHOLQR") != nullptr;
     if (const char* e = std::getenv("LAPH_QR_CYCLES")) b->qr_cycles = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("LAPH_TEST_POISON")) b->poison_ts = std::atoi(e);
-    if (const char* e = std::getenv("LAPH_FILTER_FORM")) b->prod = std::atoi(e) != 0;
+    if (const char* e = std::getenv("LAPH_FILTER_FORM")) {
+      b->prod = std::atoi(e) != 0;
+      b->qform = std::atoi(e) == 2;
+    }
+    b->qorder = laph::correction_root_order(p->deg);
     if (const char* e = std::getenv("LAPH_START_RR")) b->start_rr = std::atoi(e) != 0;
     if (const char* e = std::getenv("LAPH_TRSM")) b->use_trsm = std::atoi(e) != 0;
     b->roots = laph::leja_roots(p->deg);
@@ -1206,7 +1259,7 @@ int laph_batch_time_filter(laph_batch* b, int reps, double* ms_per_launch, doubl
     const int steps = b->deg - 1;
     double bytes = 0;
     for (int s = 1; s < b->deg; ++s) {
-      const int touches = s == 1 ? 2 : (s == 2 || b->prod ? 3 : 4);
+      const int touches = s == 1 || b->qform ? 2 : (s == 2 || b->prod ? 3 : 4);
       bytes += (double)nt * b->V3 * ((double)K * touches * 3.0 * b->esz + (K / b->bw) * 216.0);
     }
     *ms_per_launch = ms / ((double)reps * steps);

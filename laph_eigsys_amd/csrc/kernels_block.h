@@ -9,8 +9,9 @@ namespace laph {
 // out = a1 (B in) + a3 in + a2 r + b old per column, B = c_t (H - 6) + a_t;
 // coef[ts*K + col] = (a1, a2, b, a3), cab[ts] = (c_t, a_t - 6 c_t); column
 // blocks below first_blk[ts] are skipped.  old and r may be null (plain
-// recurrence without the source term; a product-form step, a (B - z) factor,
-// has a3 = -a1 z and no old vector).
+// recurrence without the source term).  A product-form step, a factor
+// a1 (B - z) with a complex shift z, has no old vector: its shift term is
+// (a3 + i b) in with a3 + i b = -a1 z.
 // Filter vectors stored as FP32 complex (bf16 = 0; bw = 64: one vector per
 // lane, 128: two per lane) or as BF16 complex (bf16 = 1; bw = 128 or 256:
 // two or four per lane); buffers block-major with bw-vector blocks.

@@ -146,7 +146,8 @@ __device__ __forceinline__ void hop(const C* __restrict__ U, const C* __restrict
 //   out = a1 * (B in) + a3 * in + a2 * r + b * old,   B = c_t (H - 6) + a_t
 // with per-column coefficients (a1, a2, b, a3) for this step; HAS_R = false
 // is the plain recurrence (no source term, one vector stream less), and a
-// null old the product-form step (no previous vector).
+// null old the product-form step (no previous vector), whose shift term is
+// complex: a3 + i b.
 template <bool HAS_R>
 __global__ __launch_bounds__(256) void k_filter_step32(const float2* __restrict__ U32,
                                                         const float2* __restrict__ in,
@@ -192,6 +193,9 @@ __global__ __launch_bounds__(256) void k_filter_step32(const float2* __restrict_
       const float2 ov = ld_nt(old + idx);
       o.x += cf.z * ov.x;
       o.y += cf.z * ov.y;
+    } else {  // imaginary part of a complex shift
+      o.x -= cf.z * ctr[c].y;
+      o.y += cf.z * ctr[c].x;
     }
     st_nt(out + idx, o);
   }
@@ -283,6 +287,11 @@ __global__ __launch_bounds__(256) void k_filter_step32x2(const float2* __restric
       o.y += cf0.z * ov.y;
       o.z += cf1.z * ov.z;
       o.w += cf1.z * ov.w;
+    } else {  // imaginary part of a complex shift
+      o.x -= cf0.z * ctr[c].y;
+      o.y += cf0.z * ctr[c].x;
+      o.z -= cf1.z * ctr[c].w;
+      o.w += cf1.z * ctr[c].z;
     }
     st_nt4(out + idx, o);
   }
@@ -399,6 +408,9 @@ __global__ __launch_bounds__(256) void k_filter_step_bf16(const float2* __restri
         const float2 oo = bf2_unpack(ov[j]);
         val.x += cf[j].z * oo.x;
         val.y += cf[j].z * oo.y;
+      } else {  // imaginary part of a complex shift
+        val.x -= cf[j].z * xc.y;
+        val.y += cf[j].z * xc.x;
       }
       o[j] = bf2_pack(val);
     }

@@ -121,12 +121,14 @@ def test_filter_block_widths(lp, gpu, storage, width, monkeypatch):
     b.close()
 
 
-@pytest.mark.parametrize("form,passes", [(1, 1), (1, 2), (1, 3), (1, 4), (0, 1), (0, 4)])
+@pytest.mark.parametrize("form,passes", [(2, 1), (2, 2), (2, 3), (2, 4), (1, 1), (1, 4), (0, 1), (0, 4)])
 def test_filter_passes(lp, gpu, form, passes, monkeypatch):
     """T_m(B)^P per cycle (correction pass + P-1 plain passes, each adding
-    gamma^(p-1) f_m in its last step) gives the dense oracle's spectrum, in
-    the product form over the roots of T_m (form 1, the default) and in the
-    three-term recurrence (form 0)."""
+    gamma^(p-1) f_m in its last step) gives the dense oracle's spectrum: in
+    the product form over the roots of T_m with the correction pass as a
+    product over the complex roots of (T_m(y) - T_m(theta)) / (y - theta)
+    applied to r (form 2, the default), with the correction pass reading r in
+    every step (form 1), and in the three-term recurrence (form 0)."""
     L, nts, nev, deg, tol = 8, 2, 40, 8, 1e-10
     data = smeared(L, 31, 0.4, nts)
     monkeypatch.setenv("LAPH_FILTER_FORM", str(form))
