@@ -88,8 +88,6 @@ static int batch_alloc(laph_batch* b) {
   LAPH_HIP(hipMalloc(&b->E, nk * sizeof(double)));
   LAPH_HIP(hipMalloc(&b->info, b->nts * sizeof(rocblas_int)));
   LAPH_HIP(hipMalloc(&b->nrm2, nk * sizeof(double)));
-  LAPH_HIP(hipMalloc(&b->lam, nk * sizeof(double)));
-  LAPH_HIP(hipMalloc(&b->cts, b->nts * sizeof(double)));
   LAPH_HIP(hipMalloc(&b->gam, nk * sizeof(double)));
   LAPH_HIP(hipMalloc(&b->coef, (size_t)std::max(1, b->deg - 1) * nk * sizeof(float4)));
   LAPH_HIP(hipMalloc(&b->coef2, (size_t)(b->deg + LAPH_MAX_PASSES - 2) * nk * sizeof(float4)));
@@ -155,7 +153,7 @@ static void batch_free(laph_batch* b) {
     if (q) (void)hipFree(q);
   void* bufs[] = {b->U64, b->U32, b->dec,  b->Unew, b->B[0], b->B[1],   b->B[2], b->R32,     b->P[0],
                   b->P[1], b->P[2], b->G,  b->G2,   b->P2,   b->G3,   b->mu,     b->E,    b->info,    b->nrm2,
-                  b->lam,  b->cts,  b->gam, b->coef, b->cab, b->first_blk, b->lock_blk, b->cn, b->coef2, b->jsweeps};
+                  b->gam, b->coef, b->cab, b->first_blk, b->lock_blk, b->cn, b->coef2, b->jsweeps};
   for (void* q : bufs)
     if (q) (void)hipFree(q);
   if (b->st) (void)hipStreamDestroy(b->st);
@@ -416,12 +414,15 @@ static int rayleigh_ritz(const View& v, Act q, bool orthonormal, bool* chol_fail
   LAPH_RB(rocblas_zgemm_strided_batched(rb, rocblas_operation_conjugate_transpose, rocblas_operation_none, q.a, b->n,
                                         q.a, &one, zc(Z), K, kk, zc(Y + q.l), K, (rocblas_stride)b->per_ts, &zero,
                                         zc(T + q.l), K, (rocblas_stride)b->per_ts, nt));
-  LAPH_HIP(launch_lap_block64(b->U64 + v.lo(), T, AY, b->p.L, K, nt, b->lock_blk + v.ln->t0, b->strip, v.ln->st));
+  // residuals of the new Ritz pairs for the next filter pass (D X is not kept:
+  // the next Rayleigh-Ritz applies D to the filtered block)
+  LAPH_HIP(launch_lap_residual64(b->U64 + v.lo(), T, b->mu + v.nk(), b->p.lambda_l, fb(b, b->R32, v.vo()), b->p.L,
+                                 K, nt, b->lock_blk + v.ln->t0, b->strip, b->bw, b->bf16, v.ln->st));
   // the hard-locked columns of X live in the old X buffer: carry them over
   if (q.l > 0)
     LAPH_HIP(hipMemcpy2DAsync(T, (size_t)K * sizeof(double2), Y, (size_t)K * sizeof(double2),
                               (size_t)q.l * sizeof(double2), (size_t)nt * b->n, hipMemcpyDeviceToDevice, v.ln->st));
-  // roles: X <- T, AX <- AY buffer (now D X), T <- old X buffer
+  // roles: X <- T, T <- old X buffer
   std::swap(v.ln->iX, v.ln->iT);
   LAPH_HIP(hipMemcpyAsync(b->h_mu.data() + v.nk(), b->mu + v.nk(), (size_t)nt * K * sizeof(double),
                           hipMemcpyDeviceToHost, v.ln->st));
@@ -466,10 +467,13 @@ static int orthonormalise_and_rr(const View& v, Act q, bool explicit_qr) {
 // its cap, 0.75 lambda_l, which a random block's largest Ritz value exceeds.
 static int start_from_block(const View& v) {
   laph_batch* b = v.b;
-  LAPH_HIP(launch_lap_block64(b->U64 + v.lo(), v.X(), v.AX(), b->p.L, b->K, v.ln->nt, b->lock_blk + v.ln->t0,
-                              b->strip, v.ln->st));
+  const size_t nk = (size_t)v.ln->nt * b->K;
+  std::fill(b->h_mu.begin() + v.nk(), b->h_mu.begin() + v.nk() + nk, 0.75 * b->p.lambda_l);
+  LAPH_HIP(hipMemcpyAsync(b->mu + v.nk(), b->h_mu.data() + v.nk(), nk * sizeof(double), hipMemcpyHostToDevice,
+                          v.ln->st));
+  LAPH_HIP(launch_lap_residual64(b->U64 + v.lo(), v.X(), b->mu + v.nk(), b->p.lambda_l, fb(b, b->R32, v.vo()),
+                                 b->p.L, b->K, v.ln->nt, b->lock_blk + v.ln->t0, b->strip, b->bw, b->bf16, v.ln->st));
   v.ln->sv64 += (long long)v.ln->nt * b->K;
-  std::fill(b->h_mu.begin() + v.nk(), b->h_mu.begin() + v.nk() + (size_t)v.ln->nt * b->K, 0.75 * b->p.lambda_l);
   return LAPH_OK;
 }
 
@@ -480,19 +484,14 @@ static int residuals_and_window(const View& v, std::vector<double>& lc, int* n_a
   laph_lane* ln = v.ln;
   const int K = b->K, nt = ln->nt;
   const double lamL = b->p.lambda_l;
-  std::vector<double> hl((size_t)nt * K), hc(nt);
   for (int t = 0; t < nt; ++t) {
-    const double* mu = &b->h_mu[v.nk() + (size_t)t * K];
     // lower end of the damped interval: largest Ritz value of -D, kept
-    // well below lambda_l so the amplification stays inside FP32 range
-    lc[t] = std::min(mu[K - 1], 0.75 * lamL);
-    hc[t] = 2.0 / (lamL - lc[t]);
-    for (int j = 0; j < K; ++j) hl[(size_t)t * K + j] = -mu[j];
+    // well below lambda_l so the amplification stays inside FP32 range (the
+    // residual kernels derive the same window from the device copy of mu)
+    lc[t] = std::min(b->h_mu[v.nk() + (size_t)t * K + K - 1], 0.75 * lamL);
   }
-  LAPH_HIP(hipMemcpyAsync(b->lam + v.nk(), hl.data(), hl.size() * sizeof(double), hipMemcpyHostToDevice, ln->st));
-  LAPH_HIP(hipMemcpyAsync(b->cts + ln->t0, hc.data(), hc.size() * sizeof(double), hipMemcpyHostToDevice, ln->st));
-  LAPH_HIP(launch_residual(v.X(), v.AX(), b->lam + v.nk(), b->cts + ln->t0, fb(b, b->R32, v.vo()), b->nrm2 + v.nk(),
-                           b->n, K, nt, ln->lock, b->bw, b->bf16, ln->st));
+  LAPH_HIP(launch_residual_norms(fb(b, b->R32, v.vo()), b->mu + v.nk(), lamL, b->nrm2 + v.nk(), b->n, K, nt, ln->lock,
+                                 b->bw, b->bf16, ln->st));
   LAPH_HIP(hipMemcpyAsync(b->h_res.data() + v.nk(), b->nrm2 + v.nk(), (size_t)nt * K * sizeof(double),
                           hipMemcpyDeviceToHost, ln->st));
   LAPH_HIP(hipStreamSynchronize(ln->st));

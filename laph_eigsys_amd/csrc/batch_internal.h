@@ -68,8 +68,6 @@ struct laph_batch {
   rocblas_int* info = nullptr;
   double* nrm2 = nullptr;    // [nts][K]
   double* cn = nullptr;      // [nts][K] column norms for CholQR scaling
-  double* lam = nullptr;     // [nts][K]
-  double* cts = nullptr;     // [nts]
   double* gam = nullptr;     // [nts][K]
   float4* coef = nullptr;    // [deg-1][nts][K]
   float4* coef2 = nullptr;   // [deg - 1 + LAPH_MAX_PASSES - 1][nts][K] plain passes

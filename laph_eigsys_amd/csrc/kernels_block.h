@@ -22,11 +22,17 @@ hipError_t launch_filter_step(const float2* U32, const void* in, const void* r, 
 // W: sweep strip width (divides L; L = plain x-major order)
 hipError_t launch_lap_block64(const double2* U64, const double2* v, double2* out, int L, int K, int nts,
                               const int* first_blk, int W, hipStream_t st);
-// R = AX - X diag(lam), r32 = c_t R, nrm2 = column sums of |R|^2, for the
-// columns >= first_col (a multiple of 64).
-hipError_t launch_residual(const double2* X, const double2* AX, const double* lam, const double* cts,
-                           void* r, double* nrm2, int n, int K, int nts, int first_col, int bw, int bf16,
-                           hipStream_t st);
+// Residuals of the Ritz pairs from the FP64 stencil: r = c_t (D X + X diag(mu))
+// in the filter storage (block-major, bw-vector blocks, FP32 or BF16), mu the
+// Ritz values of -D ([nts][K]), c_t = 2 / (lamL - min(mu[t][K-1], 0.75 lamL)),
+// for the 64-column blocks >= first_blk[ts].
+hipError_t launch_lap_residual64(const double2* U64, const double2* X, const double* mu, double lamL, void* r,
+                                 int L, int K, int nts, const int* first_blk, int W, int bw, int bf16,
+                                 hipStream_t st);
+// nrm2 = column sums of |D X + X diag(mu)|^2 from that r, for the columns
+// >= first_col (a multiple of 64).
+hipError_t launch_residual_norms(const void* r, const double* mu, double lamL, double* nrm2, int n, int K, int nts,
+                                 int first_col, int bw, int bf16, hipStream_t st);
 // Column-major, phase-fixed copy of the first nev columns of one timeslice.
 // psi(r) = sum_x |V_{x+r} V_x^H|_F for r in {0, 2, ..}^3 below L, V in the
 // eigenvector-file layout (n x nev column-major); psi has ((L+1)/2)^3 entries

@@ -463,42 +463,74 @@ __global__ __launch_bounds__(256) void k_lap_block64(const double2* __restrict__
   }
 }
 
-// R = AX - X diag(lam); r32 = c_t * R (B-space residual, FP32); nrm2[ts][col] += |R|^2,
-// for columns >= first_col.  grid (row chunks, active 64-column blocks, ts);
-// each thread sums the rows of one column within its chunk.
-__global__ __launch_bounds__(256) void k_residual(const double2* __restrict__ X,
-                                                   const double2* __restrict__ AX,
-                                                   const double* __restrict__ lam,
-                                                   const double* __restrict__ cts,
-                                                   void* __restrict__ rbuf,
-                                                   double* __restrict__ nrm2, int n, int K,
-                                                   int rows_per_block, int first_col, int bw, int bf16) {
-  const int ts = blockIdx.z;
-  const int col = first_col + blockIdx.y * 64 + (threadIdx.x & 63);
-  const int r0 = blockIdx.x * rows_per_block + (threadIdx.x >> 6);
-  const int r1 = min(n, (int)(blockIdx.x + 1) * rows_per_block);
-  const double l = lam[(size_t)ts * K + col];
-  const double c = cts[ts];
-  const size_t off = (size_t)ts * n * K;
-  double acc = 0.0;
-  for (int row = r0; row < r1; row += 4) {
-    const size_t idx = off + (size_t)row * K + col;
-    const double2 a = AX[idx], x = X[idx];
-    const double rx = a.x - l * x.x, ry = a.y - l * x.y;
-    acc += rx * rx + ry * ry;
-    const size_t ri = (((size_t)ts * (K / bw) + col / bw) * n + row) * bw + col % bw;
-    const float2 rv = make_float2((float)(c * rx), (float)(c * ry));
+// c_t = 2 / (lambda_l - lc_t) of a timeslice, lc_t = min(largest Ritz value
+// of -D, 0.75 lambda_l): the scale of B = c_t D + a_t in the current window.
+__device__ __forceinline__ double window_scale(const double* __restrict__ mu_ts, int K, double lamL) {
+  return 2.0 / (lamL - fmin(mu_ts[K - 1], 0.75 * lamL));
+}
+
+// Residuals of the Ritz pairs straight from the FP64 stencil: R = D X + X
+// diag(mu) (mu the Ritz values of -D), stored as r = c_t R in the filter's
+// block-major storage [ts][col/bw][row][bw] (FP32, or BF16 pairs), for the
+// 64-column blocks >= first_blk[ts].  D X itself is not kept.
+__global__ __launch_bounds__(256) void k_lap_residual64(const double2* __restrict__ U64,
+                                                         const double2* __restrict__ v,
+                                                         const double* __restrict__ mu, double lamL,
+                                                         void* __restrict__ rbuf, int L, int K, int nts,
+                                                         const int* __restrict__ first_blk, int W, int bw,
+                                                         int bf16) {
+  const int V3 = L * L * L;
+  int ts, cb, sb;
+  unit_map((V3 + 3) / 4, K / 64, nts, ts, cb, sb);
+  if (cb < first_blk[ts]) return;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int si = sb * 4 + wave;
+  if (si >= V3) return;
+  const int s = strip_site(si, L, W);
+  const int col = cb * 64 + (threadIdx.x & 63);
+  const size_t vo = (size_t)ts * V3 * 3 * K;
+  double2 acc[3], ctr[3];
+  hop<double2>(U64 + (size_t)ts * V3 * 27, v + vo, s, L, K, col, acc, ctr);
+  const double* m = mu + (size_t)ts * K;
+  const double c = window_scale(m, K, lamL), mc = m[col];
+  const size_t n = (size_t)3 * V3;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float2 rv = make_float2((float)(c * (acc[a].x + (mc - 6.0) * ctr[a].x)),
+                                  (float)(c * (acc[a].y + (mc - 6.0) * ctr[a].y)));
+    const size_t ri = (((size_t)ts * (K / bw) + col / bw) * n + 3 * (size_t)s + a) * bw + col % bw;
     if (bf16)
       static_cast<unsigned*>(rbuf)[ri] = bf2_pack(rv);
     else
       static_cast<float2*>(rbuf)[ri] = rv;
   }
+}
+
+// nrm2[ts][col] += |R|^2 summed over the rows of one chunk, from r = c_t R in
+// the filter storage, for the columns >= first_col.  grid (row chunks,
+// active 64-column groups, ts); each thread sums the rows of one column.
+__global__ __launch_bounds__(256) void k_residual_norms(const void* __restrict__ rbuf,
+                                                         const double* __restrict__ mu, double lamL,
+                                                         double* __restrict__ nrm2, int n, int K,
+                                                         int rows_per_block, int first_col, int bw, int bf16) {
+  const int ts = blockIdx.z;
+  const int col = first_col + blockIdx.y * 64 + (threadIdx.x & 63);
+  const int r0 = blockIdx.x * rows_per_block + (threadIdx.x >> 6);
+  const int r1 = min(n, (int)(blockIdx.x + 1) * rows_per_block);
+  const double c = window_scale(mu + (size_t)ts * K, K, lamL);
+  const size_t base = ((size_t)ts * (K / bw) + col / bw) * n * bw + col % bw;
+  double acc = 0.0;
+  for (int row = r0; row < r1; row += 4) {
+    const size_t ri = base + (size_t)row * bw;
+    const float2 rv = bf16 ? bf2_unpack(static_cast<const unsigned*>(rbuf)[ri]) : static_cast<const float2*>(rbuf)[ri];
+    acc += (double)rv.x * rv.x + (double)rv.y * rv.y;
+  }
   __shared__ double part[256];
-  part[threadIdx.x] = acc;
+  part[threadIdx.x] = acc / (c * c);
   __syncthreads();
   if (threadIdx.x < 64) {
-    double s = part[threadIdx.x] + part[threadIdx.x + 64] + part[threadIdx.x + 128] + part[threadIdx.x + 192];
-    atomicAdd(&nrm2[(size_t)ts * K + col], s);
+    const double t = part[threadIdx.x] + part[threadIdx.x + 64] + part[threadIdx.x + 128] + part[threadIdx.x + 192];
+    atomicAdd(&nrm2[(size_t)ts * K + col], t);
   }
 }
 
@@ -621,14 +653,20 @@ hipError_t launch_lap_block64(const double2* U64, const double2* v, double2* out
   return hipGetLastError();
 }
 
-hipError_t launch_residual(const double2* X, const double2* AX, const double* lam, const double* cts,
-                           void* r, double* nrm2, int n, int K, int nts, int first_col, int bw, int bf16,
-                           hipStream_t st) {
+hipError_t launch_lap_residual64(const double2* U64, const double2* X, const double* mu, double lamL, void* r,
+                                 int L, int K, int nts, const int* first_blk, int W, int bw, int bf16,
+                                 hipStream_t st) {
+  k_lap_residual64<<<site_grid(L, K, nts), 256, 0, st>>>(U64, X, mu, lamL, r, L, K, nts, first_blk, W, bw, bf16);
+  return hipGetLastError();
+}
+
+hipError_t launch_residual_norms(const void* r, const double* mu, double lamL, double* nrm2, int n, int K, int nts,
+                                 int first_col, int bw, int bf16, hipStream_t st) {
   const int w = K - first_col;
   k_zero_cols<<<(nts * w + 255) / 256, 256, 0, st>>>(nrm2, nts, K, first_col);
   const int rows = 256;
   dim3 g((n + rows - 1) / rows, w / 64, nts);
-  k_residual<<<g, 256, 0, st>>>(X, AX, lam, cts, r, nrm2, n, K, rows, first_col, bw, bf16);
+  k_residual_norms<<<g, 256, 0, st>>>(r, mu, lamL, nrm2, n, K, rows, first_col, bw, bf16);
   return hipGetLastError();
 }
 
