@@ -43,7 +43,7 @@ PEAK_HBM = 8.0e12
 # --pmc FETCH_SIZE / WRITE_SIZE on the isolated replays of --profile-filter,
 # each scaled by the same counter on a 1 GiB device copy (python bench.py
 # --summarize-pmc ...); None = not measured for this build.
-TRAFFIC_BYTES_PER_LAUNCH = None  # re-measured for the correction pass without r reads
+TRAFFIC_BYTES_PER_LAUNCH = 34955853363  # profiles/filter_step_32x32x32x64_k256_pmc.md
 
 
 def main():
