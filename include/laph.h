@@ -182,6 +182,7 @@ typedef struct laph_batch laph_batch;
  *   LAPH_QR_CYCLES=n        cycles (and start block) orthonormalised by explicit CholQR2 (0)
  *   LAPH_START_RR=1         Rayleigh-Ritz on the Gaussian start block before the first cycle (off: the
  *                           first cycle filters the block as it is)
+ *   LAPH_AMP_CAP=x          bound on T_DEG(theta_max)^P, the amplification of one cycle (1e10)
  *   LAPH_SHIFTED_CHOLQR=1   every CholQR pass starts with a shifted one
  *   LAPH_TRSM=1             Rayleigh-Ritz through triangular solves with the Gram's Cholesky factor
  *                           (else its explicit inverse and GEMMs)
@@ -199,7 +200,8 @@ typedef struct laph_batch_opts {
   unsigned long long seed;
   int warm_start;    /* 1: start from the block proposed by laph_net_propose, when one was */
   int filter_passes; /* applications of T_DEG(B) per Rayleigh-Ritz cycle once the window has
-                        settled: 1 .. 6 (0 = 4) */
+                        settled: 1 .. 6 (0 = 5), fewer while T_DEG(theta_max)^P would exceed
+                        1e10 (a wide window) */
   int filter_storage; /* filter vectors in HBM: 0 FP32 complex, 1 BF16 complex (half the
                          bytes; arithmetic stays FP32, the FP64 Rayleigh-Ritz and residual
                          test are unchanged, so only the cycle count can differ) */

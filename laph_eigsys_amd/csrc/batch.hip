@@ -855,20 +855,21 @@ __global__ void k_combine_gamma(double2* __restrict__ X, const E* __restrict__ W
 }
 
 // Applications of T_m(B) in this cycle of a lane: the requested count, cut
-// so that the largest amplification T_m(theta_max)^P stays <= 1e8, with
-// theta_max = 1 + c lambda_c >= the image of the lowest eigenvalue of -D
-// (-D >= 0).  A column whose Ritz value sits at the window edge (theta = 1,
-// no scaling) grows its components along the lowest eigenvectors by that
-// factor: beyond ~1e8 such buffer columns become numerically parallel in
-// FP64, and a wide early window would overflow FP32.
+// so that the largest amplification T_m(theta_max)^P stays <= amp_cap (1e10),
+// with theta_max = 1 + c lambda_c >= the image of the lowest eigenvalue of
+// -D (-D >= 0).  A column whose Ritz value sits at the window edge (theta =
+// 1, no scaling) grows its components along the lowest eigenvectors by that
+// factor: far beyond the cap such buffer columns become numerically parallel
+// in FP64 (their FP32 rounding then swamps their own direction), and a wide
+// early window would overflow FP32.
 static int filter_passes(const laph_batch* b, const std::vector<double>& lc) {
-  const int want = std::min(b->o.filter_passes > 0 ? b->o.filter_passes : 4, LAPH_MAX_PASSES);
+  const int want = std::min(b->o.filter_passes > 0 ? b->o.filter_passes : 5, LAPH_MAX_PASSES);
   double amax = 1.0;
   for (double l : lc) {
     const double th = 1.0 + 2.0 * l / (b->p.lambda_l - l);
     amax = std::max(amax, std::cosh(b->deg * std::acosh(th)));
   }
-  const int pmax = amax > 1.0 ? (int)std::floor(std::log(1e8) / std::log(amax)) : want;
+  const int pmax = amax > 1.0 ? (int)std::floor(std::log(b->amp_cap) / std::log(amax)) : want;
   return std::max(1, std::min(want, pmax));
 }
 
@@ -1165,6 +1166,7 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     b->qorder = laph::correction_root_order(p->deg);
     if (const char* e = std::getenv("LAPH_START_RR")) b->start_rr = std::atoi(e) != 0;
     if (const char* e = std::getenv("LAPH_TRSM")) b->use_trsm = std::atoi(e) != 0;
+    if (const char* e = std::getenv("LAPH_AMP_CAP")) b->amp_cap = std::max(10.0, std::atof(e));
     b->roots = laph::leja_roots(p->deg);
     b->strip = strip > 0 ? strip : p->L;
     b->strip_fixed = strip > 0;

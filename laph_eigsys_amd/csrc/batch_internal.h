@@ -107,6 +107,7 @@ struct laph_batch {
   bool qform = true;         // correction pass as a product over the roots of q_theta (LAPH_FILTER_FORM=2,
                              // the default; 1: product form reading r in every step)
   std::vector<int> qorder;   // order of those roots (indices k = 1 .. deg-1)
+  double amp_cap = 1e10;     // bound on T_m(theta_max)^P of a cycle (LAPH_AMP_CAP)
   bool use_trsm = false;     // triangular solves instead of the explicit inverse in Rayleigh-Ritz (LAPH_TRSM=1)
   bool start_rr = false;     // Rayleigh-Ritz on the Gaussian start block before the first cycle (LAPH_START_RR=1)
   bool force_shift = false;  // every CholQR pass starts with a shifted one (LAPH_SHIFTED_CHOLQR)
