@@ -6,29 +6,30 @@
 // multiple of 64), in residual-correction form so the FP32 filter acts on a
 // shrinking quantity.  For a Ritz pair (x, theta) of B = c D + a with
 // r = (B - theta) x, the degree-m Chebyshev filter satisfies exactly
-//     T_m(B) x = T_m(theta) x + e_m,
-//     e_0 = 0, e_1 = r, e_{k+1} = 2 B e_k - e_{k-1} + 2 T_k(theta) r,
+//     T_m(B) x = T_m(theta) x + e_m,   e_m = q(B) r,
+//     q(y) = (T_m(y) - T_m(theta)) / (y - theta),
 // so only e (of size |r|) is computed in FP32; x stays FP64 and the FP32
-// rounding error scales with the residual.  The recurrence is carried in the
-// scaled form f_k = e_k / s_k, s_k = max(|T_k(theta)|, 1), which keeps every
-// column O(|r|) and turns each step into
-//     f_{k+1} = a1 (B f_k) + a2 r + b f_{k-1}     (per-column a1, a2, b)
-// and the filtered vector into y = (T_m(theta)/s_m) x + f_m.
+// rounding error scales with the residual.  Both polynomials are applied in
+// product form, one factor a1 (B - z) per step (a complex shift for q when
+// theta > 1), so a step reads one vector and writes one; the columns are
+// carried scaled to stay O(|r|).  T_m(B) is applied P times per cycle (a
+// correction pass, then plain passes that add gamma^(p-1) f_m in their last
+// step), with P cut so that T_m(theta_max)^P stays below the amplification cap.
 //
-// The timeslices are split into two lanes, each driven by its own host
-// thread on its own stream, so that one lane's HBM-bound filter overlaps the
-// other lane's MFMA-bound dense steps.  Per cycle (every timeslice of a lane
-// batched):
-//   1. FP32 filter, m-1 fused stencil steps (hipGraph, one launch per step
-//      covering every active vector block of every timeslice)
-//   2. Y = gamma X + F (FP64)
-//   3. CholQR2 in FP64 (Gram zgemm, zpotrf, ztrsm; all strided-batched)
-//   4. AY = D Y (FP64 blocked stencil)
-//   5. Rayleigh-Ritz in FP64: H = Y^H AY (and, after the first cycles, the
-//      Gram G = Y^H Y instead of an explicit CholQR2), zheevd, X = Y Q,
-//      AX = D X by the FP64 blocked stencil
-//   6. residuals R = AX - X Lambda in FP64; lock the leading columns with
-//      |R_j| <= tol; window lower bound = largest Ritz value of -D.
+// Per cycle (every timeslice of a lane batched; one lane by default, more
+// host threads and streams with LAPH_LANES):
+//   1. FP32 filter (hipGraphs, one launch per step covering every active
+//      vector block of every timeslice)
+//   2. Y = gamma^P X + W (FP64), projection against the hard-locked columns
+//   3. AY = D Y (FP64 blocked stencil); Gram G = Y^H Y and H = Y^H AY
+//   4. Rayleigh-Ritz in FP64 on the non-orthonormal block: Cholesky of the
+//      equilibrated Gram, its explicit inverse, zheevd, X = Y U^-1 Q
+//      (CholQR2 and shifted CholQR as fallbacks)
+//   5. r = c_t (D X + X Lambda) from the FP64 stencil of the rotated block and
+//      its column norms; lock the leading columns with |R_j| <= tol; window
+//      lower bound = largest Ritz value of -D.
+// The first cycle starts from the seeded Gaussian block itself (every column
+// at the window edge), without a Rayleigh-Ritz step.
 // The window map is B = c_t D + a_t with c_t = 2/(lambda_l - lambda_c,t),
 // a_t = 1 + c_t lambda_c,t (shell_matop.cpp:49-50) and lambda_c,t adapted per
 // timeslice; eigenvalues come straight from the FP64 Rayleigh-Ritz, so
