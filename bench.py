@@ -38,6 +38,7 @@ L, LT, EPS, NEV, DEG, TOL, NSUB = 32, 64, 0.3, 220, 16, 1e-10, 256
 SEED = 1227
 LAMBDA_L = 12.0
 PEAK_HBM = 8.0e12
+PEAK_FP64_MFMA = 78.6e12  # dense FP64 matrix peak of the MI355X
 # HBM bytes per filter-step launch (all columns active, the 64 timeslices of
 # the default single lane, 128-vector blocks, y-strips of 4) from rocprofv3
 # --pmc FETCH_SIZE / WRITE_SIZE on the isolated replays of --profile-filter,
@@ -61,6 +62,9 @@ def main():
                     help="per-kernel durations from a rocprofv3 --kernel-trace database (no GPU use)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="HBM bytes per filter launch from two rocprofv3 --pmc runs of --profile-filter")
+    ap.add_argument("--summarize-mfma", metavar="PMC_CSV",
+                    help="FP64 matrix-core rate per kernel from a rocprofv3 --pmc run with "
+                         "SQ_INSTS_VALU_MFMA_MOPS_F64")
     ap.add_argument("--workload", choices=("c3", "c2", "c4"), default="c3",
                     help="c3: the headline (32^3x64, N_ev 220); c2: 24^3x48 N_ev 120, every timeslice through the "
                          "plain drop-in and the batched solver; c4: 48^3x96 N_ev 200 in HBM-sized groups")
@@ -75,6 +79,9 @@ def main():
         return
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc, args.last)
+        return
+    if args.summarize_mfma:
+        summarize_mfma(args.summarize_mfma)
         return
     if args.stream_ceiling:
         print(json.dumps(stream_ceiling()))
@@ -284,6 +291,30 @@ def _solve_phases(rows):
     print(f"last solve: {span:.1f} ms from its start block to its last residual")
     for ph, (wall, busy, cnt) in sorted(agg.items(), key=lambda kv: -kv[1][0]):
         print(f"  {ph:62s} {wall:8.1f} ms wall {busy:8.1f} ms kernels {cnt:6d} dispatches")
+
+
+def summarize_mfma(path):
+    """Per kernel: FP64 matrix flops (512 per SQ_INSTS_VALU_MFMA_MOPS_F64
+    unit), summed dispatch time and the resulting TFLOP/s against the FP64
+    matrix peak (other counters in the file are ignored)."""
+    import csv
+    disp = {}
+    for r in csv.DictReader(open(path)):
+        d = disp.setdefault(r["Dispatch_Id"], {"name": r["Kernel_Name"],
+                                               "ns": int(r["End_Timestamp"]) - int(r["Start_Timestamp"])})
+        d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    agg = {}
+    for d in disp.values():
+        a = agg.setdefault(d["name"][:70], [0, 0.0, 0.0, 0.0, 0.0])
+        a[0] += 1
+        a[1] += d["ns"]
+        a[2] += 512.0 * d.get("SQ_INSTS_VALU_MFMA_MOPS_F64", 0.0)
+    print(f"{'kernel':70s} {'calls':>6s} {'ms':>9s} {'TFLOP':>9s} {'TFLOP/s':>8s} {'of peak':>8s}")
+    for name, (n, ns, fl, _, _) in sorted(agg.items(), key=lambda kv: -kv[1][2]):
+        if fl <= 0:
+            continue
+        rate = fl / (ns * 1e-9) / 1e12
+        print(f"{name:70s} {n:6d} {ns / 1e6:9.2f} {fl / 1e12:9.3f} {rate:8.2f} {rate / (PEAK_FP64_MFMA / 1e12):8.3f}")
 
 
 def summarize_pmc(fetch_csv, write_csv, last):
