@@ -106,14 +106,20 @@ def main():
     cfg = lp.synth_config(SEED, EPS, L, nts)
     vol = 72 * L ** 3
     b = lp.Batch(L, DEG, LAMBDA_L, 0.0, nts, NEV, nsub=NSUB, tol=TOL, seed=SEED)
+    t_field = time.time() - t0
+    sync = device_sync()
+    sync()
+    t1 = time.perf_counter()
     for t in range(nts):
         b.set_timeslice(t, cfg[t * vol:(t + 1) * vol], 0.62, 0.62, 3)
-    print(f"[bench] setup {time.time() - t0:.1f}s (field + HYP of {nts} timeslices)", file=sys.stderr, flush=True)
+    sync()
+    t_hyp = time.perf_counter() - t1  # host-to-device copy, map to links and HYP of every timeslice
+    print(f"[bench] setup {time.time() - t0:.1f}s (field {t_field:.1f}s, upload + map + HYP of {nts} timeslices "
+          f"{t_hyp:.2f}s)", file=sys.stderr, flush=True)
 
     for w in range(args.warmup):
         st = b.solve()
         print(f"[bench] warmup {w}: {st.seconds:.2f}s cycles={st.iterations}", file=sys.stderr, flush=True)
-    sync = device_sync()
     sync()
     tot = {"seconds": 0.0, "filter": 0.0, "fbytes": 0.0, "flaunch": 0, "cycles": 0, "maxres": 0.0}
     t_start = time.perf_counter()
@@ -184,6 +190,7 @@ def main():
                        "lambda_c": lamC, "cycles": plain_cycles},
         "speedup_vs_plain_fp64": (value / plain_value) if plain_value else None,
         "max_residual": tot["maxres"], "cycles": tot["cycles"],
+        "upload_map_hyp_seconds": t_hyp,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
