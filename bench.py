@@ -4,10 +4,10 @@ mixed-precision batched solver vs the plain FP64 drop-in, on one MI355X.
 A step is one batched eigensolve of all 64 timeslices of the seeded
 synthetic configuration (U = exp(i 0.3 H), HYP 0.62/0.62/3 applied on the
 device beforehand; smeared links resident in HBM when the timed region
-starts): FP32 Chebyshev filter of degree 16 (T_16(B)^5 per cycle once the window has
-settled) on 64- or 128-vector blocks, FP64 CholQR2 / Rayleigh-Ritz / residual
-locking, until every timeslice has 220 locked pairs with
-|D v - lambda v| <= 1e-10.
+starts): FP32 Chebyshev filter in residual-correction form (two cycles of
+degree 16, then one filter per cycle of the degree the window allows, up to
+72) on 64- or 128-vector blocks, FP64 Rayleigh-Ritz / residual locking, until
+every timeslice has 220 locked pairs with |D v - lambda v| <= 1e-10.
 
 Printed JSON (one line) also carries:
   roofline      the FP32 filter stencil (dominant kernel): algorithmic bytes
@@ -181,7 +181,7 @@ def main():
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32 filter + fp64 CholQR2/RR/lock", "data": "synthetic",
         "config": {"workload": f"{L}^3x{nts} eps={EPS} HYP 0.62/0.62/3, N_ev={NEV}, Chebyshev degree {DEG}, "
-                               f"T_{DEG}(B)^5 per cycle, vector block {st.filter_block}, "
+                               f"one filter pass per cycle of degree <= 72 after two of degree {DEG}, vector block {st.filter_block}, "
                                f"sweep strip {st.filter_strip}, search space {NSUB}, "
                                f"residual <= {TOL}",
                    "lattice": f"{L}^3x{nts}", "nev": NEV, "deg": DEG, "tol": TOL, "parallelism": "none (1 GPU)"},

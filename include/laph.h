@@ -182,7 +182,9 @@ typedef struct laph_batch laph_batch;
  *   LAPH_QR_CYCLES=n        cycles (and start block) orthonormalised by explicit CholQR2 (0)
  *   LAPH_START_RR=1         Rayleigh-Ritz on the Gaussian start block before the first cycle (off: the
  *                           first cycle filters the block as it is)
- *   LAPH_AMP_CAP=x          bound on T_DEG(theta_max)^P, the amplification of one cycle (1e10)
+ *   LAPH_AMP_CAP=x          bound on the amplification T_d(theta_max) (or T_DEG(theta_max)^P) of a cycle (1e10)
+ *   LAPH_FILTER_DEGREE=d    highest per-cycle filter degree (72); 0: applications of T_DEG(B) per cycle
+ *                           (up to 5, or filter_passes)
  *   LAPH_SHIFTED_CHOLQR=1   every CholQR pass starts with a shifted one
  *   LAPH_TRSM=1             Rayleigh-Ritz through triangular solves with the Gram's Cholesky factor
  *                           (else its explicit inverse and GEMMs)
@@ -199,9 +201,10 @@ typedef struct laph_batch_opts {
   int max_iter;      /* filter / Rayleigh-Ritz cycles (0 = 60) */
   unsigned long long seed;
   int warm_start;    /* 1: start from the block proposed by laph_net_propose, when one was */
-  int filter_passes; /* applications of T_DEG(B) per Rayleigh-Ritz cycle once the window has
-                        settled: 1 .. 6 (0 = 5), fewer while T_DEG(theta_max)^P would exceed
-                        1e10 (a wide window) */
+  int filter_passes; /* 0: after two warm-up cycles of T_DEG(B), one Chebyshev filter per cycle of
+                        the highest degree (a multiple of 4, at most 72) whose amplification
+                        T_d(theta_max) stays below 1e10; 1 .. 6: up to that many applications of
+                        T_DEG(B) per cycle, fewer while T_DEG(theta_max)^P would exceed 1e10 */
   int filter_storage; /* filter vectors in HBM: 0 FP32 complex, 1 BF16 complex (half the
                          bytes; arithmetic stays FP32, the FP64 Rayleigh-Ritz and residual
                          test are unchanged, so only the cycle count can differ) */

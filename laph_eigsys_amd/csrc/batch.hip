@@ -90,7 +90,8 @@ static int batch_alloc(laph_batch* b) {
   LAPH_HIP(hipMalloc(&b->info, b->nts * sizeof(rocblas_int)));
   LAPH_HIP(hipMalloc(&b->nrm2, nk * sizeof(double)));
   LAPH_HIP(hipMalloc(&b->gam, nk * sizeof(double)));
-  LAPH_HIP(hipMalloc(&b->coef, (size_t)std::max(1, b->deg - 1) * nk * sizeof(float4)));
+  const int csteps = std::max(1, std::max(b->deg, b->fdeg_max) - 1);  // correction steps of the largest degree
+  LAPH_HIP(hipMalloc(&b->coef, (size_t)csteps * nk * sizeof(float4)));
   LAPH_HIP(hipMalloc(&b->coef2, (size_t)(b->deg + LAPH_MAX_PASSES - 2) * nk * sizeof(float4)));
   LAPH_HIP(hipMalloc(&b->cab, b->nts * sizeof(float2)));
   LAPH_HIP(hipMalloc(&b->first_blk, b->nts * sizeof(int)));
@@ -98,7 +99,7 @@ static int batch_alloc(laph_batch* b) {
   LAPH_HIP(hipMalloc(&b->cn, nk * sizeof(double)));
   LAPH_HIP(hipMalloc(&b->jsweeps, b->nts * sizeof(rocblas_int)));
   // defined contents for the width autotuner's first replays
-  LAPH_HIP(hipMemset(b->coef, 0, (size_t)std::max(1, b->deg - 1) * nk * sizeof(float4)));
+  LAPH_HIP(hipMemset(b->coef, 0, (size_t)csteps * nk * sizeof(float4)));
   LAPH_HIP(hipMemset(b->coef2, 0, (size_t)(b->deg + LAPH_MAX_PASSES - 2) * nk * sizeof(float4)));
   LAPH_HIP(hipMemset(b->cab, 0, b->nts * sizeof(float2)));
   LAPH_HIP(hipMemset(b->first_blk, 0, b->nts * sizeof(int)));
@@ -138,6 +139,8 @@ static void batch_free(laph_batch* b) {
   for (auto& ln : b->lanes) {
     if (ln.st) (void)hipStreamSynchronize(ln.st);
     if (ln.graph) (void)hipGraphExecDestroy(ln.graph);
+    for (auto& kv : ln.gdeg)
+      if (kv.second) (void)hipGraphExecDestroy(kv.second);
     for (auto g : ln.gplain)
       if (g) (void)hipGraphExecDestroy(g);
     for (auto e : ln.ev)
@@ -186,6 +189,7 @@ struct View {
   size_t nk() const { return (size_t)ln->t0 * b->K; }
   size_t vo() const { return (size_t)ln->t0 * b->per_ts; }
   size_t lo() const { return (size_t)ln->t0 * b->V3 * 27; }
+  int m() const { return ln->m > 0 ? ln->m : b->deg; }  // filter degree of the current cycle
 };
 
 static int dense_sync_info(const View& v, const char* what) {
@@ -613,12 +617,13 @@ static std::vector<int> correction_root_order(int m) {
 // cos phi (complex phi for theta > 1: complex shifts).  Step st makes
 // h_st = q_st(B) r / sigma_st from h_{st-1} (h_0 = r), sigma_st = max(|q_st(theta)|, 1)
 // and sigma_{m-1} = sm, so the pass ends with e_m / sm without reading r again.
-static void correction_q_coefficients(const laph_batch* b, double th, int m, double sm, float4* co, size_t stride) {
+static void correction_q_coefficients(const std::vector<int>& order, double th, int m, double sm, float4* co,
+                                      size_t stride) {
   const std::complex<double> phi = std::acos(std::complex<double>(th, 0.0));
   std::complex<double> qk(1.0, 0.0);
   double sig = 1.0;
   for (int st = 1; st < m; ++st) {
-    const std::complex<double> x = std::cos(phi + 2.0 * M_PI * b->qorder[st - 1] / m);
+    const std::complex<double> x = std::cos(phi + 2.0 * M_PI * order[st - 1] / m);
     qk *= 2.0 * (th - x);
     const double sn = st == m - 1 ? sm : std::max(std::abs(qk), 1.0);
     const double a1 = 2.0 * sig / sn;
@@ -631,7 +636,7 @@ static void correction_q_coefficients(const laph_batch* b, double th, int m, dou
 static int upload_coefficients(const View& v, const std::vector<double>& lc, int passes) {
   laph_batch* b = v.b;
   laph_lane* ln = v.ln;
-  const int K = b->K, nt = ln->nt, m = b->deg;
+  const int K = b->K, nt = ln->nt, m = v.m();
   const int n2 = m + LAPH_MAX_PASSES - 2;  // plain-pass tables: m-1 shared steps + one last step per pass
   const double lamL = b->p.lambda_l;
   std::vector<float4> co((size_t)(m - 1) * nt * K), co2((size_t)n2 * nt * K);
@@ -648,10 +653,17 @@ static int upload_coefficients(const View& v, const std::vector<double>& lc, int
     const double* mu = &b->h_mu[v.nk() + (size_t)t * K];
     for (int j = 0; j < K; ++j) {
       const double th = 1.0 + c * (lc[t] - mu[j]);  // Ritz value in B
+      if (b->fdeg_max > 0) {  // one correction pass of this cycle's degree
+        const double tm = std::real(std::cos((double)m * std::acos(std::complex<double>(th, 0.0))));
+        const double sm = std::max(std::fabs(tm), 1.0);
+        correction_q_coefficients(b->qorders[m], th, m, sm, &co[(size_t)t * K + j], (size_t)nt * K);
+        ga[(size_t)t * K + j] = tm / sm;
+        continue;
+      }
       if (b->prod) {
         product_coefficients(b, th, m, &co[(size_t)t * K + j], &co2[(size_t)t * K + j], (size_t)nt * K,
                              &ga[(size_t)t * K + j], passes, tau.data(), s.data());
-        if (b->qform) correction_q_coefficients(b, th, m, s[m], &co[(size_t)t * K + j], (size_t)nt * K);
+        if (b->qform) correction_q_coefficients(b->qorders[m], th, m, s[m], &co[(size_t)t * K + j], (size_t)nt * K);
         continue;
       }
       tau[0] = 1.0;
@@ -689,7 +701,7 @@ static int upload_coefficients(const View& v, const std::vector<double>& lc, int
   for (int st = 1; st < m; ++st)
     LAPH_HIP(hipMemcpyAsync(b->coef + (size_t)(st - 1) * b->nts * K + v.nk(), co.data() + (size_t)(st - 1) * nt * K,
                             (size_t)nt * K * sizeof(float4), hipMemcpyHostToDevice, ln->st));
-  for (int st = 1; st <= n2; ++st)
+  for (int st = 1; b->fdeg_max == 0 && st <= n2; ++st)  // plain passes exist only at the batch's degree
     LAPH_HIP(hipMemcpyAsync(b->coef2 + (size_t)(st - 1) * b->nts * K + v.nk(), co2.data() + (size_t)(st - 1) * nt * K,
                             (size_t)nt * K * sizeof(float4), hipMemcpyHostToDevice, ln->st));
   LAPH_HIP(hipMemcpyAsync(b->gam + v.nk(), ga.data(), ga.size() * sizeof(double), hipMemcpyHostToDevice, ln->st));
@@ -701,7 +713,7 @@ static int upload_coefficients(const View& v, const std::vector<double>& lc, int
 // The m-1 FP32 filter steps of a lane; buffers rotate R32 -> P0 -> P1 -> P2 -> P0 ...
 static int enqueue_filter(const View& v) {
   laph_batch* b = v.b;
-  const int m = b->deg, K = b->K, nt = v.ln->nt, L = b->p.L;
+  const int m = v.m(), K = b->K, nt = v.ln->nt, L = b->p.L;
   const size_t step = (size_t)b->nts * K;
   const size_t vo = v.vo();
   void* R = fb(b, b->R32, vo);
@@ -722,7 +734,7 @@ static int enqueue_filter(const View& v) {
 // rotate over the other three buffers q[] (R32 is free by then).
 static void plain_buffers(const View& v, void** pf, void* q[3]) {
   laph_batch* b = v.b;
-  const int m = b->deg;
+  const int m = v.m();
   void* all[4] = {b->R32, b->P[0], b->P[1], b->P[2]};
   void* f = b->P[(m - 2) % (b->prod ? 2 : 3)];
   int k = 0;
@@ -810,10 +822,13 @@ static int capture(const View& v, hipGraphExec_t* exec, const std::function<int(
 // T_m(B)^passes x / s_m^passes = gamma^passes x + w_passes.
 static int run_filter(const View& v, int passes, void** w) {
   laph_lane* ln = v.ln;
-  if (!ln->graph) LAPH_TRY(capture(v, &ln->graph, [&] { return enqueue_filter(v); }));
+  // the batch's own degree uses ln->graph; other degrees (per-cycle degree
+  // mode) a graph each, captured on first use
+  hipGraphExec_t* g = v.m() == v.b->deg ? &ln->graph : &ln->gdeg[v.m()];
+  if (!*g) LAPH_TRY(capture(v, g, [&] { return enqueue_filter(v); }));
   for (int p = 2; p <= passes; ++p)
     if (!ln->gplain[p - 2]) LAPH_TRY(capture(v, &ln->gplain[p - 2], [&] { return enqueue_plain_pass(v, p); }));
-  LAPH_HIP(hipGraphLaunch(ln->graph, ln->st));
+  LAPH_HIP(hipGraphLaunch(*g, ln->st));
   void *f, *q[3];
   plain_buffers(v, &f, q);
   *w = f;
@@ -874,6 +889,18 @@ static int filter_passes(const laph_batch* b, const std::vector<double>& lc) {
   return std::max(1, std::min(want, pmax));
 }
 
+// Per-cycle degree mode: the filter degree of this cycle of a lane, the
+// largest multiple of 4 up to fdeg_max with T_d(theta_max) <= amp_cap (one
+// application per cycle); the wide window of the first cycle gets a low
+// degree, the settled windows a high one.
+static int cycle_degree(const laph_batch* b, const std::vector<double>& lc) {
+  double pmax = 0.0;  // largest acosh(theta_max) over the lane's timeslices
+  for (double l : lc) pmax = std::max(pmax, std::acosh(1.0 + 2.0 * l / (b->p.lambda_l - l)));
+  int d = pmax > 0.0 ? (int)std::floor(std::acosh(b->amp_cap) / pmax) : b->fdeg_max;
+  d = std::min(d, b->fdeg_max) / 4 * 4;
+  return std::max(d, 4);
+}
+
 // Solve loop of one lane (runs on its own host thread).
 // Largest residual of the wanted pairs not yet converged over the lane's
 // timeslices (0 when all are).
@@ -917,15 +944,19 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
   const int max_iter = b->o.max_iter > 0 ? b->o.max_iter : 60;
   int it = 0;
   for (it = 0; it < max_iter && active > 0; ++it) {
-    // one filter pass while the window settles, then T_m(B)^P per cycle
-    const int passes = it < b->warm_cycles ? 1 : filter_passes(b, lc);
+    // one filter pass while the window settles, then T_m(B)^P per cycle; in
+    // the per-cycle degree mode one pass of the degree the window allows
+    const bool warmup = it < b->warm_cycles;
+    ln->m = b->fdeg_max > 0 && !warmup ? cycle_degree(b, lc) : b->deg;
+    const int passes = b->fdeg_max > 0 || warmup ? 1 : filter_passes(b, lc);
+    const int m = ln->m;
     LAPH_TRY(upload_coefficients(v, lc, passes));
     for (int t = 0; t < nt; ++t) {
       const int nl = b->nlock[ln->t0 + t];
       if (nl >= b->nev) continue;
       const int cols = K - (nl / 64) * 64;
       const int nblk = (cols + b->bw - 1) / b->bw;  // blocks the kernel visits (links once each)
-      ln->sv32 += (long long)(b->deg - 1) * cols;
+      ln->sv32 += (long long)(m - 1) * cols;
       // algorithmic bytes (vector touches per step): product form, correction
       // step 1 reads r and writes e_2 (2), later steps read e_k and r and
       // write (3); plain-pass steps read and write (2), the last one also
@@ -933,19 +964,19 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
       // e_{k-1} (4), plain-pass steps after the first the previous vector.
       // A vector touch is 3 colours x esz bytes per site; links 27 x 8 B per
       // site per vector block
-      for (int s = 1; s < b->deg; ++s) {
+      for (int s = 1; s < m; ++s) {
         const int touches = s == 1 || b->qform ? 2 : (s == 2 || b->prod ? 3 : 4);
         ln->fbytes += (double)b->V3 * ((double)cols * touches * 3.0 * b->esz + nblk * 216.0);
       }
       for (int p = 2; p <= passes; ++p) {
-        ln->sv32 += (long long)b->deg * cols;
-        for (int s = 1; s <= b->deg; ++s) {
-          const int touches = (s == 1 || b->prod ? 2 : 3) + (s == b->deg ? 1 : 0);
+        ln->sv32 += (long long)m * cols;
+        for (int s = 1; s <= m; ++s) {
+          const int touches = (s == 1 || b->prod ? 2 : 3) + (s == m ? 1 : 0);
           ln->fbytes += (double)b->V3 * ((double)cols * touches * 3.0 * b->esz + nblk * 216.0);
         }
       }
     }
-    ln->flaunch += (b->deg - 1) + (long long)(passes - 1) * b->deg;
+    ln->flaunch += (m - 1) + (long long)(passes - 1) * m;
     void* w = nullptr;
     LAPH_HIP(hipEventRecord(ln->ev[0], ln->st));
     LAPH_TRY(run_filter(v, passes, &w));
@@ -972,8 +1003,8 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
       const double worst = worst_residual(b, ln);
       const double* mu = &b->h_mu[0];
       const double* rs = &b->h_res[0];
-      std::fprintf(stderr, "[laph] cycle %d: passes %d  window %.4f  mu[nev-1] %.5f  mu[K-1] %.4f  res[0] %.2e  "
-                   "res[nev-1] %.2e  worst %.2e  locked %d  lock %d\n", it, passes, lc[0], mu[b->nev - 1], mu[K - 1],
+      std::fprintf(stderr, "[laph] cycle %d: degree %d passes %d  window %.4f  mu[nev-1] %.5f  mu[K-1] %.4f  res[0] %.2e  "
+                   "res[nev-1] %.2e  worst %.2e  locked %d  lock %d\n", it, m, passes, lc[0], mu[b->nev - 1], mu[K - 1],
                    std::sqrt(rs[0]), std::sqrt(rs[b->nev - 1]), worst, b->nlock[0], ln->lock);
     }
     float a = 0, c = 0;
@@ -1016,6 +1047,7 @@ static int time_correction_pass(laph_batch* b, int reps, float* ms) {
 // LAPH_BW / LAPH_STRIP fix either.
 static int autotune_block_width(laph_batch* b) {
   laph_lane* ln = &b->lanes[0];
+  ln->m = 0;  // layouts are timed at the batch's degree
   LAPH_HIP(hipMemsetAsync(b->first_blk + ln->t0, 0, ln->nt * sizeof(int), ln->st));
   int cand[2];
   block_widths(b, cand);
@@ -1164,7 +1196,14 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
       b->prod = std::atoi(e) != 0;
       b->qform = std::atoi(e) == 2;
     }
-    b->qorder = laph::correction_root_order(p->deg);
+    // automatic passes: one pass per cycle of the degree the window allows (up
+    // to 72); an explicit pass count: the batch's degree, up to that many passes
+    b->fdeg_max = o->filter_passes > 0 ? 0 : 72;
+    if (const char* e = std::getenv("LAPH_FILTER_DEGREE")) b->fdeg_max = std::max(0, std::min(256, std::atoi(e)));
+    if (b->fdeg_max > 0) b->prod = b->qform = true;  // one correction pass per cycle: product over q's roots
+    b->qorders.resize(std::max(p->deg, b->fdeg_max) + 1);
+    for (int d = 2; d < (int)b->qorders.size(); ++d)
+      if (d == p->deg || (b->fdeg_max > 0 && d % 4 == 0)) b->qorders[d] = laph::correction_root_order(d);
     if (const char* e = std::getenv("LAPH_START_RR")) b->start_rr = std::atoi(e) != 0;
     if (const char* e = std::getenv("LAPH_TRSM")) b->use_trsm = std::atoi(e) != 0;
     if (const char* e = std::getenv("LAPH_AMP_CAP")) b->amp_cap = std::max(10.0, std::atof(e));
@@ -1245,6 +1284,7 @@ int laph_batch_time_filter(laph_batch* b, int reps, double* ms_per_launch, doubl
   LAPH_HIP(hipSetDevice(b->p.device));
   return laph::guarded([&]() -> int {
     laph_lane* ln = &b->lanes[0];
+    ln->m = 0;  // the batch's degree
     const laph::View v{b, ln};
     const int K = b->K, nt = ln->nt;
     // every column of every timeslice of lane 0 active

@@ -4,6 +4,7 @@
 
 #include <rocblas/rocblas.h>
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -29,7 +30,9 @@ struct laph_lane {
   rocblas_handle rb = nullptr;
   int iX = 0, iAX = 1, iT = 2;  // roles of the three FP64 block buffers
   int lock = 0;                 // hard-locked columns, common to the lane
-  hipGraphExec_t graph = nullptr;  // correction-form filter pass
+  hipGraphExec_t graph = nullptr;  // correction-form filter pass of the batch's degree
+  std::map<int, hipGraphExec_t> gdeg;  // correction passes by degree (per-cycle degree mode)
+  int m = 0;                    // filter degree of the current cycle (0: the batch's degree)
   hipGraphExec_t gplain[LAPH_MAX_PASSES - 1] = {};  // plain passes 2 .. LAPH_MAX_PASSES
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   // per-solve statistics
@@ -104,9 +107,12 @@ struct laph_batch {
   bool prod = true;          // product-form filter over Leja-ordered roots of T_m; LAPH_FILTER_FORM=0:
                              // three-term recurrence
   std::vector<double> roots; // roots of T_deg in the order the product form applies them
+  int fdeg_max = 72;         // per-cycle filter degree up to this, one application per cycle after the
+                             // warm-up cycles (LAPH_FILTER_DEGREE); 0: the batch's degree and up to
+                             // filter_passes applications per cycle (an explicit filter_passes selects it)
+  std::vector<std::vector<int>> qorders;  // order of the correction roots by degree (index = degree)
   bool qform = true;         // correction pass as a product over the roots of q_theta (LAPH_FILTER_FORM=2,
                              // the default; 1: product form reading r in every step)
-  std::vector<int> qorder;   // order of those roots (indices k = 1 .. deg-1)
   double amp_cap = 1e10;     // bound on T_m(theta_max)^P of a cycle (LAPH_AMP_CAP)
   bool use_trsm = false;     // triangular solves instead of the explicit inverse in Rayleigh-Ritz (LAPH_TRSM=1)
   bool start_rr = false;     // Rayleigh-Ritz on the Gaussian start block before the first cycle (LAPH_START_RR=1)

@@ -143,6 +143,26 @@ def test_filter_passes(lp, gpu, form, passes, monkeypatch):
     b.close()
 
 
+@pytest.mark.parametrize("fdeg", [24, 72, 0])
+def test_filter_degree_per_cycle(lp, gpu, fdeg, monkeypatch):
+    """Automatic passes (filter_passes = 0): after the warm-up cycles one
+    filter per cycle of the highest degree (a multiple of 4 up to
+    LAPH_FILTER_DEGREE) whose amplification stays under the cap; 0 selects
+    T_DEG(B) applied up to 5 times per cycle.  All give the dense oracle's
+    spectrum."""
+    L, nts, nev, deg, tol = 8, 2, 40, 8, 1e-10
+    data = smeared(L, 313, 0.4, nts)
+    monkeypatch.setenv("LAPH_FILTER_DEGREE", str(fdeg))
+    b = lp.Batch(L, deg, 12.0, 0.0, nts, nev, nsub=64, tol=tol, seed=7, device=gpu)
+    for t, (ts, _) in enumerate(data):
+        b.set_timeslice(t, ts, 0.62, 0.62, 3)
+    assert b.solve().converged == nts
+    for t, (_, U) in enumerate(data):
+        ev, V, ph, res = b.get(t)
+        check_pairs(ev, V, res, N.dense_D(U.reshape(L, L, L, 3, 3, 3)), nev, tol)
+    b.close()
+
+
 def test_shifted_cholqr_path(lp, gpu, monkeypatch):
     """Every Cholesky QR pass preceded by the shifted pass (the fallback for
     numerically singular Grams): same spectrum as the dense oracle."""
