@@ -62,6 +62,9 @@ def main():
                     help="per-kernel durations from a rocprofv3 --kernel-trace database (no GPU use)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="HBM bytes per filter launch from two rocprofv3 --pmc runs of --profile-filter")
+    ap.add_argument("--gemm-solutions", action="store_true",
+                    help="only time every rocBLAS solution for the solver's large GEMM shapes "
+                         "(laph_eigsys_amd/gemm_solutions)")
     ap.add_argument("--summarize-mfma", metavar="PMC_CSV",
                     help="FP64 matrix-core rate per kernel from a rocprofv3 --pmc run with "
                          "SQ_INSTS_VALU_MFMA_MOPS_F64")
@@ -86,6 +89,10 @@ def main():
     if args.stream_ceiling:
         print(json.dumps(stream_ceiling()))
         return
+    if args.gemm_solutions:
+        import subprocess
+        exe = os.path.join(ROOT, "laph_eigsys_amd", "gemm_solutions")
+        sys.exit(subprocess.run([exe]).returncode)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); timeslices are "
               "independent, so N GPUs would each run their own Lt/N share with no exchange.", file=sys.stderr)
