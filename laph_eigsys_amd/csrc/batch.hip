@@ -617,13 +617,22 @@ static std::vector<int> correction_root_order(int m) {
 // cos phi (complex phi for theta > 1: complex shifts).  Step st makes
 // h_st = q_st(B) r / sigma_st from h_{st-1} (h_0 = r), sigma_st = max(|q_st(theta)|, 1)
 // and sigma_{m-1} = sm, so the pass ends with e_m / sm without reading r again.
-static void correction_q_coefficients(const std::vector<int>& order, double th, int m, double sm, float4* co,
-                                      size_t stride) {
+// cos and sin of 2 pi k / m for the roots of q, in their application order.
+static std::vector<std::pair<double, double>> correction_root_rotations(int m) {
+  std::vector<std::pair<double, double>> r;
+  for (int k : correction_root_order(m)) r.emplace_back(std::cos(2.0 * M_PI * k / m), std::sin(2.0 * M_PI * k / m));
+  return r;
+}
+
+static void correction_q_coefficients(const std::vector<std::pair<double, double>>& rot, double th, int m, double sm,
+                                      float4* co, size_t stride) {
   const std::complex<double> phi = std::acos(std::complex<double>(th, 0.0));
+  const std::complex<double> sphi = std::sin(phi);
   std::complex<double> qk(1.0, 0.0);
   double sig = 1.0;
   for (int st = 1; st < m; ++st) {
-    const std::complex<double> x = std::cos(phi + 2.0 * M_PI * order[st - 1] / m);
+    // x = cos(phi + 2 pi k / m), with cos(phi) = theta
+    const std::complex<double> x = th * rot[st - 1].first - sphi * rot[st - 1].second;
     qk *= 2.0 * (th - x);
     const double sn = st == m - 1 ? sm : std::max(std::abs(qk), 1.0);
     const double a1 = 2.0 * sig / sn;
@@ -639,64 +648,72 @@ static int upload_coefficients(const View& v, const std::vector<double>& lc, int
   const int K = b->K, nt = ln->nt, m = v.m();
   const int n2 = m + LAPH_MAX_PASSES - 2;  // plain-pass tables: m-1 shared steps + one last step per pass
   const double lamL = b->p.lambda_l;
-  std::vector<float4> co((size_t)(m - 1) * nt * K), co2((size_t)n2 * nt * K);
+  std::vector<float4> co((size_t)(m - 1) * nt * K), co2(b->fdeg_max > 0 ? 0 : (size_t)n2 * nt * K);
   std::vector<double> ga((size_t)nt * K);
   std::vector<float2> cab(nt);
   std::vector<int> fb(nt);
-  std::vector<double> tau(m + 1), s(m + 1);
-  for (int t = 0; t < nt; ++t) {
-    const double c = 2.0 / (lamL - lc[t]);
-    const double a = 1.0 + c * lc[t];
-    cab[t] = make_float2((float)c, (float)(a - 6.0 * c));
-    const int nl = b->nlock[ln->t0 + t];
-    fb[t] = nl >= b->nev ? K / 64 : nl / 64;
-    const double* mu = &b->h_mu[v.nk() + (size_t)t * K];
-    for (int j = 0; j < K; ++j) {
-      const double th = 1.0 + c * (lc[t] - mu[j]);  // Ritz value in B
-      if (b->fdeg_max > 0) {  // one correction pass of this cycle's degree
-        const double tm = std::real(std::cos((double)m * std::acos(std::complex<double>(th, 0.0))));
-        const double sm = std::max(std::fabs(tm), 1.0);
-        correction_q_coefficients(b->qorders[m], th, m, sm, &co[(size_t)t * K + j], (size_t)nt * K);
-        ga[(size_t)t * K + j] = tm / sm;
-        continue;
-      }
-      if (b->prod) {
-        product_coefficients(b, th, m, &co[(size_t)t * K + j], &co2[(size_t)t * K + j], (size_t)nt * K,
-                             &ga[(size_t)t * K + j], passes, tau.data(), s.data());
-        if (b->qform) correction_q_coefficients(b->qorders[m], th, m, s[m], &co[(size_t)t * K + j], (size_t)nt * K);
-        continue;
-      }
-      tau[0] = 1.0;
-      tau[1] = th;
-      for (int k = 1; k < m; ++k) tau[k + 1] = 2.0 * th * tau[k] - tau[k - 1];
-      for (int k = 0; k <= m; ++k) s[k] = std::max(std::fabs(tau[k]), 1.0);
-      for (int st = 1; st < m; ++st) {
-        float4 q;
-        if (st == 1) {
-          q = make_float4((float)(2.0 / s[2]), (float)(2.0 * tau[1] / s[2]), 0.f, 0.f);
-        } else if (st == 2) {
-          q = make_float4((float)(2.0 * s[2] / s[3]), (float)((2.0 * tau[2] - 1.0) / s[3]), 0.f, 0.f);
-        } else {
-          q = make_float4((float)(2.0 * s[st] / s[st + 1]), (float)(2.0 * tau[st] / s[st + 1]),
-                          (float)(-s[st - 1] / s[st + 1]), 0.f);
+  // the timeslices' tables are independent: filled by a few host threads
+  auto fill = [&](int tb, int te) {
+    std::vector<double> tau(m + 1), s(m + 1);
+    for (int t = tb; t < te; ++t) {
+      const double c = 2.0 / (lamL - lc[t]);
+      const double a = 1.0 + c * lc[t];
+      cab[t] = make_float2((float)c, (float)(a - 6.0 * c));
+      const int nl = b->nlock[ln->t0 + t];
+      fb[t] = nl >= b->nev ? K / 64 : nl / 64;
+      const double* mu = &b->h_mu[v.nk() + (size_t)t * K];
+      for (int j = 0; j < K; ++j) {
+        const double th = 1.0 + c * (lc[t] - mu[j]);  // Ritz value in B
+        if (b->fdeg_max > 0) {  // one correction pass of this cycle's degree
+          const double tm = std::real(std::cos((double)m * std::acos(std::complex<double>(th, 0.0))));
+          const double sm = std::max(std::fabs(tm), 1.0);
+          correction_q_coefficients(b->qrot[m], th, m, sm, &co[(size_t)t * K + j], (size_t)nt * K);
+          ga[(size_t)t * K + j] = tm / sm;
+          continue;
         }
-        co[((size_t)(st - 1) * nt + t) * K + j] = q;
+        if (b->prod) {
+          product_coefficients(b, th, m, &co[(size_t)t * K + j], &co2[(size_t)t * K + j], (size_t)nt * K,
+                               &ga[(size_t)t * K + j], passes, tau.data(), s.data());
+          if (b->qform) correction_q_coefficients(b->qrot[m], th, m, s[m], &co[(size_t)t * K + j], (size_t)nt * K);
+          continue;
+        }
+        tau[0] = 1.0;
+        tau[1] = th;
+        for (int k = 1; k < m; ++k) tau[k + 1] = 2.0 * th * tau[k] - tau[k - 1];
+        for (int k = 0; k <= m; ++k) s[k] = std::max(std::fabs(tau[k]), 1.0);
+        for (int st = 1; st < m; ++st) {
+          float4 q;
+          if (st == 1) {
+            q = make_float4((float)(2.0 / s[2]), (float)(2.0 * tau[1] / s[2]), 0.f, 0.f);
+          } else if (st == 2) {
+            q = make_float4((float)(2.0 * s[2] / s[3]), (float)((2.0 * tau[2] - 1.0) / s[3]), 0.f, 0.f);
+          } else {
+            q = make_float4((float)(2.0 * s[st] / s[st + 1]), (float)(2.0 * tau[st] / s[st + 1]),
+                            (float)(-s[st - 1] / s[st + 1]), 0.f);
+          }
+          co[((size_t)(st - 1) * nt + t) * K + j] = q;
+        }
+        // plain pass p on w = w_{p-1}: h_k = T_k(B) w / s_k with h_0 = w,
+        // h_1 = B w / s_1, h_{k+1} = (2 s_k/s_{k+1}) B h_k - (s_{k-1}/s_{k+1}) h_{k-1};
+        // its last step also adds gamma^(p-1) f_m (source slot), giving w_p
+        const double g = tau[m] / s[m];
+        co2[((size_t)0 * nt + t) * K + j] = make_float4((float)(1.0 / s[1]), 0.f, 0.f, 0.f);
+        for (int st = 2; st < m; ++st)
+          co2[((size_t)(st - 1) * nt + t) * K + j] =
+              make_float4((float)(2.0 * s[st - 1] / s[st]), 0.f, (float)(-s[st - 2] / s[st]), 0.f);
+        double gp = g;
+        for (int pp = 2; pp <= LAPH_MAX_PASSES; ++pp, gp *= g)
+          co2[((size_t)(m - 1 + pp - 2) * nt + t) * K + j] =
+              make_float4((float)(2.0 * s[m - 1] / s[m]), (float)gp, (float)(-s[m - 2] / s[m]), 0.f);
+        ga[(size_t)t * K + j] = std::pow(g, passes);
       }
-      // plain pass p on w = w_{p-1}: h_k = T_k(B) w / s_k with h_0 = w,
-      // h_1 = B w / s_1, h_{k+1} = (2 s_k/s_{k+1}) B h_k - (s_{k-1}/s_{k+1}) h_{k-1};
-      // its last step also adds gamma^(p-1) f_m (source slot), giving w_p
-      const double g = tau[m] / s[m];
-      co2[((size_t)0 * nt + t) * K + j] = make_float4((float)(1.0 / s[1]), 0.f, 0.f, 0.f);
-      for (int st = 2; st < m; ++st)
-        co2[((size_t)(st - 1) * nt + t) * K + j] =
-            make_float4((float)(2.0 * s[st - 1] / s[st]), 0.f, (float)(-s[st - 2] / s[st]), 0.f);
-      double gp = g;
-      for (int pp = 2; pp <= LAPH_MAX_PASSES; ++pp, gp *= g)
-        co2[((size_t)(m - 1 + pp - 2) * nt + t) * K + j] =
-            make_float4((float)(2.0 * s[m - 1] / s[m]), (float)gp, (float)(-s[m - 2] / s[m]), 0.f);
-      ga[(size_t)t * K + j] = std::pow(g, passes);
     }
-  }
+  };
+  const int nth = std::max(1, std::min({8, nt, (int)std::thread::hardware_concurrency()}));
+  std::vector<std::thread> pool;
+  for (int h = 1; h < nth; ++h) pool.emplace_back(fill, h * nt / nth, (h + 1) * nt / nth);
+  fill(0, nt / nth);
+  for (auto& th : pool) th.join();
   // coefficient table of the lane: [step][lane timeslice][K] at offset step*nts*K + t0*K
   for (int st = 1; st < m; ++st)
     LAPH_HIP(hipMemcpyAsync(b->coef + (size_t)(st - 1) * b->nts * K + v.nk(), co.data() + (size_t)(st - 1) * nt * K,
@@ -1201,9 +1218,9 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     b->fdeg_max = o->filter_passes > 0 ? 0 : 72;
     if (const char* e = std::getenv("LAPH_FILTER_DEGREE")) b->fdeg_max = std::max(0, std::min(256, std::atoi(e)));
     if (b->fdeg_max > 0) b->prod = b->qform = true;  // one correction pass per cycle: product over q's roots
-    b->qorders.resize(std::max(p->deg, b->fdeg_max) + 1);
-    for (int d = 2; d < (int)b->qorders.size(); ++d)
-      if (d == p->deg || (b->fdeg_max > 0 && d % 4 == 0)) b->qorders[d] = laph::correction_root_order(d);
+    b->qrot.resize(std::max(p->deg, b->fdeg_max) + 1);
+    for (int d = 2; d < (int)b->qrot.size(); ++d)
+      if (d == p->deg || (b->fdeg_max > 0 && d % 4 == 0)) b->qrot[d] = laph::correction_root_rotations(d);
     if (const char* e = std::getenv("LAPH_START_RR")) b->start_rr = std::atoi(e) != 0;
     if (const char* e = std::getenv("LAPH_TRSM")) b->use_trsm = std::atoi(e) != 0;
     if (const char* e = std::getenv("LAPH_AMP_CAP")) b->amp_cap = std::max(10.0, std::atof(e));

@@ -110,7 +110,8 @@ struct laph_batch {
   int fdeg_max = 72;         // per-cycle filter degree up to this, one application per cycle after the
                              // warm-up cycles (LAPH_FILTER_DEGREE); 0: the batch's degree and up to
                              // filter_passes applications per cycle (an explicit filter_passes selects it)
-  std::vector<std::vector<int>> qorders;  // order of the correction roots by degree (index = degree)
+  // cos, sin of 2 pi k / m of the correction roots in application order, by degree m
+  std::vector<std::vector<std::pair<double, double>>> qrot;
   bool qform = true;         // correction pass as a product over the roots of q_theta (LAPH_FILTER_FORM=2,
                              // the default; 1: product form reading r in every step)
   double amp_cap = 1e10;     // bound on T_m(theta_max)^P of a cycle (LAPH_AMP_CAP)
