@@ -174,6 +174,8 @@ typedef struct laph_batch laph_batch;
  *   LAPH_LANES=1..4         host lanes (threads, streams) the timeslices are split over (1)
  *   LAPH_BW=64|128|256      vectors per filter block (else timed on the first solve)
  *   LAPH_STRIP=w            sweep strip width of the stencils, w divides L (else timed)
+ *   LAPH_ZRUN=1|2|4|8       z-sites per wavefront of the FP32 128-vector filter step, dividing L
+ *                           (default 4, else 2, else 1)
  *   LAPH_FILTER_FORM=0|1|2  T_m(B) applied as a product over its Leja-ordered roots, the correction
  *                           pass as a product over the roots of (T_m(y) - T_m(theta)) / (y - theta)
  *                           applied to r (2); the same with the correction pass reading r in every
@@ -230,6 +232,8 @@ typedef struct laph_batch_stats {
                               LAPH_STRIP fixes it */
   int recoveries;          /* times the active columns of failing timeslices were re-seeded
                               after even shifted Cholesky QR failed (a non-finite block) */
+  int filter_zrun;         /* z-sites per wavefront of the FP32 128-vector filter step (4 when
+                              it divides L, else 2 or 1; LAPH_ZRUN fixes it) */
 } laph_batch_stats;
 
 int laph_batch_create(laph_batch** out, const laph_params* params, const laph_batch_opts* opts);

@@ -741,7 +741,7 @@ static int enqueue_filter(const View& v) {
     void* out = fb(b, b->P[(st - 1) % nb], vo);
     LAPH_HIP(launch_filter_step(b->U32 + v.lo(), in, b->qform ? nullptr : R, old, out,
                                 b->coef + (st - 1) * step + v.nk(),
-                                b->cab + v.ln->t0, b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, b->strip, v.ln->st));
+                                b->cab + v.ln->t0, b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, b->strip, b->zrun, v.ln->st));
   }
   return LAPH_OK;
 }
@@ -816,7 +816,7 @@ static int enqueue_plain_pass(const View& v, int p) {
     const bool last = st == m;
     const float4* co = b->coef2 + (size_t)(last ? m - 1 + p - 2 : st - 1) * step + v.nk();
     LAPH_HIP(launch_filter_step(b->U32 + v.lo(), in, last ? f : nullptr, old, out, co, b->cab + v.ln->t0,
-                                b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, b->strip, v.ln->st));
+                                b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, b->strip, b->zrun, v.ln->st));
   }
   return LAPH_OK;
 }
@@ -1154,6 +1154,7 @@ static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
   s.iterations = cycles;
   s.filter_block = b->bw;
   s.filter_strip = b->strip;
+  s.filter_zrun = b->bf16 == 0 && b->bw == 128 ? b->zrun : 1;
   s.converged = conv;
   s.max_residual = maxres;
   s.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1193,6 +1194,12 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     strip = std::atoi(e);
     LAPH_CHECK(strip >= 1 && p->L % strip == 0, LAPH_ERR_INVALID, "LAPH_STRIP=%s must divide L=%d", e, p->L);
   }
+  int zrun = p->L % 4 == 0 ? 4 : (p->L % 2 == 0 ? 2 : 1);  // LAPH_ZRUN: z-sites per filter wavefront
+  if (const char* e = std::getenv("LAPH_ZRUN")) {
+    zrun = std::atoi(e);
+    LAPH_CHECK((zrun == 1 || zrun == 2 || zrun == 4 || zrun == 8) && p->L % zrun == 0, LAPH_ERR_INVALID,
+               "LAPH_ZRUN=%s: 1, 2, 4 or 8 dividing L=%d", e, p->L);
+  }
   return laph::guarded([&]() -> int {
     laph_batch* b = new laph_batch();
     b->p = *p;
@@ -1224,6 +1231,7 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     if (const char* e = std::getenv("LAPH_START_RR")) b->start_rr = std::atoi(e) != 0;
     if (const char* e = std::getenv("LAPH_TRSM")) b->use_trsm = std::atoi(e) != 0;
     if (const char* e = std::getenv("LAPH_AMP_CAP")) b->amp_cap = std::max(10.0, std::atof(e));
+    b->zrun = zrun;
     b->roots = laph::leja_roots(p->deg);
     b->strip = strip > 0 ? strip : p->L;
     b->strip_fixed = strip > 0;

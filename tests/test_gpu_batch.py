@@ -121,6 +121,36 @@ def test_filter_block_widths(lp, gpu, storage, width, monkeypatch):
     b.close()
 
 
+@pytest.mark.parametrize("zrun", [1, 2, 4, 8])
+def test_filter_z_runs(lp, gpu, zrun, monkeypatch):
+    """The FP32 128-vector filter step with each wavefront walking zrun sites
+    of a z-line (z-neighbour rows shifted through registers; the run's first
+    and last sites take their outer z-neighbour across the periodic edge at
+    L = 8) gives the dense oracle's spectrum, in both strip orders."""
+    L, nts, nev, deg, tol = 8, 2, 40, 8, 1e-10
+    data = smeared(L, 57, 0.4, nts)
+    monkeypatch.setenv("LAPH_BW", "128")
+    monkeypatch.setenv("LAPH_ZRUN", str(zrun))
+    for strip in (4, 8):
+        monkeypatch.setenv("LAPH_STRIP", str(strip))
+        b = lp.Batch(L, deg, 12.0, 0.0, nts, nev, nsub=128, tol=tol, seed=4, device=gpu)
+        for t, (ts, _) in enumerate(data):
+            b.set_timeslice(t, ts, 0.62, 0.62, 3)
+        st = b.solve()
+        assert st.converged == nts and st.filter_zrun == zrun and st.filter_strip == strip
+        for t, (_, U) in enumerate(data):
+            ev, V, ph, res = b.get(t)
+            check_pairs(ev, V, res, N.dense_D(U.reshape(L, L, L, 3, 3, 3)), nev, tol)
+        b.close()
+
+
+def test_filter_z_run_rejected(lp, gpu, monkeypatch):
+    """A z-run that does not divide L is refused at creation."""
+    monkeypatch.setenv("LAPH_ZRUN", "4")
+    with pytest.raises(lp.LaphError):
+        lp.Batch(6, 8, 12.0, 0.0, 1, 10, nsub=64, tol=1e-10, seed=1, device=gpu)
+
+
 @pytest.mark.parametrize("form,passes", [(2, 1), (2, 2), (2, 3), (2, 4), (1, 1), (1, 4), (0, 1), (0, 4)])
 def test_filter_passes(lp, gpu, form, passes, monkeypatch):
     """T_m(B)^P per cycle (correction pass + P-1 plain passes, each adding
