@@ -174,8 +174,8 @@ typedef struct laph_batch laph_batch;
  *   LAPH_LANES=1..4         host lanes (threads, streams) the timeslices are split over (1)
  *   LAPH_BW=64|128|256      vectors per filter block (else timed on the first solve)
  *   LAPH_STRIP=w            sweep strip width of the stencils, w divides L (else timed)
- *   LAPH_ZRUN=1|2|4|8       z-sites per wavefront of the FP32 128-vector filter step, dividing L
- *                           (default 4, else 2, else 1)
+ *   LAPH_ZRUN=0|1|2|4|8     FP32 128-vector filter step in work-groups of 4 y-rows (L a multiple of 4):
+ *                           z-sites per wavefront, dividing L (8, else 4); 0: one site per wavefront
  *   LAPH_FILTER_FORM=0|1|2  T_m(B) applied as a product over its Leja-ordered roots, the correction
  *                           pass as a product over the roots of (T_m(y) - T_m(theta)) / (y - theta)
  *                           applied to r (2); the same with the correction pass reading r in every
@@ -228,12 +228,13 @@ typedef struct laph_batch_stats {
                               BF16: 128 or 256), chosen by timing on the first solve
                               unless LAPH_BW fixes it */
   int filter_strip;        /* sweep strip width of the stencil kernels (L = x-major
-                              planes, or 4), chosen with the block width unless
+                              planes, 4 or 8), chosen with the block width unless
                               LAPH_STRIP fixes it */
   int recoveries;          /* times the active columns of failing timeslices were re-seeded
                               after even shifted Cholesky QR failed (a non-finite block) */
-  int filter_zrun;         /* z-sites per wavefront of the FP32 128-vector filter step (4 when
-                              it divides L, else 2 or 1; LAPH_ZRUN fixes it) */
+  int filter_zrun;         /* FP32 128-vector filter step in work-groups of 4 y-rows: z-sites
+                              per wavefront (8 or 4; LAPH_ZRUN fixes it); 0 = one site per
+                              wavefront (other layouts, L not a multiple of 4, LAPH_ZRUN=0) */
 } laph_batch_stats;
 
 int laph_batch_create(laph_batch** out, const laph_params* params, const laph_batch_opts* opts);

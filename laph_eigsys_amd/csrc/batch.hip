@@ -741,7 +741,7 @@ static int enqueue_filter(const View& v) {
     void* out = fb(b, b->P[(st - 1) % nb], vo);
     LAPH_HIP(launch_filter_step(b->U32 + v.lo(), in, b->qform ? nullptr : R, old, out,
                                 b->coef + (st - 1) * step + v.nk(),
-                                b->cab + v.ln->t0, b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, b->strip, b->zrun, v.ln->st));
+                                b->cab + v.ln->t0, b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, b->strip, b->strip % 4 == 0 ? b->zrun : 0, v.ln->st));
   }
   return LAPH_OK;
 }
@@ -816,7 +816,7 @@ static int enqueue_plain_pass(const View& v, int p) {
     const bool last = st == m;
     const float4* co = b->coef2 + (size_t)(last ? m - 1 + p - 2 : st - 1) * step + v.nk();
     LAPH_HIP(launch_filter_step(b->U32 + v.lo(), in, last ? f : nullptr, old, out, co, b->cab + v.ln->t0,
-                                b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, b->strip, b->zrun, v.ln->st));
+                                b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, b->strip, b->strip % 4 == 0 ? b->zrun : 0, v.ln->st));
   }
   return LAPH_OK;
 }
@@ -1058,7 +1058,7 @@ static int time_correction_pass(laph_batch* b, int reps, float* ms) {
 }
 
 // Layout of the filter: block width (two candidates of the storage format)
-// and sweep strip width (L, or 4 when it divides L), each pair timed on
+// and sweep strip width (L, or 4 or 8 when they divide L), each pair timed on
 // lane 0's timeslices with every column active (the kernels only stream the
 // buffers; their contents do not change the time); the fastest is kept.
 // LAPH_BW / LAPH_STRIP fix either.
@@ -1076,6 +1076,7 @@ static int autotune_block_width(laph_batch* b) {
   } else {
     strips.push_back(b->p.L);
     if (b->p.L % 4 == 0 && b->p.L > 4) strips.push_back(4);
+    if (b->p.L % 8 == 0 && b->p.L > 8) strips.push_back(8);
   }
   float best = 0;
   int wbest = widths[0], sbest = strips[0];
@@ -1154,7 +1155,7 @@ static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
   s.iterations = cycles;
   s.filter_block = b->bw;
   s.filter_strip = b->strip;
-  s.filter_zrun = b->bf16 == 0 && b->bw == 128 ? b->zrun : 1;
+  s.filter_zrun = b->bf16 == 0 && b->bw == 128 && b->strip % 4 == 0 ? b->zrun : 0;
   s.converged = conv;
   s.max_residual = maxres;
   s.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1194,11 +1195,12 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     strip = std::atoi(e);
     LAPH_CHECK(strip >= 1 && p->L % strip == 0, LAPH_ERR_INVALID, "LAPH_STRIP=%s must divide L=%d", e, p->L);
   }
-  int zrun = p->L % 4 == 0 ? 4 : (p->L % 2 == 0 ? 2 : 1);  // LAPH_ZRUN: z-sites per filter wavefront
+  // LAPH_ZRUN: z-sites per wavefront of the y-row work-group filter step (0: one site per wavefront)
+  int zrun = p->L % 4 != 0 ? 0 : (p->L % 8 == 0 ? 8 : 4);
   if (const char* e = std::getenv("LAPH_ZRUN")) {
     zrun = std::atoi(e);
-    LAPH_CHECK((zrun == 1 || zrun == 2 || zrun == 4 || zrun == 8) && p->L % zrun == 0, LAPH_ERR_INVALID,
-               "LAPH_ZRUN=%s: 1, 2, 4 or 8 dividing L=%d", e, p->L);
+    LAPH_CHECK(zrun == 0 || ((zrun == 1 || zrun == 2 || zrun == 4 || zrun == 8) && p->L % zrun == 0 && p->L % 4 == 0),
+               LAPH_ERR_INVALID, "LAPH_ZRUN=%s: 0, or 1, 2, 4 or 8 dividing L=%d (a multiple of 4)", e, p->L);
   }
   return laph::guarded([&]() -> int {
     laph_batch* b = new laph_batch();

@@ -93,7 +93,7 @@ struct laph_batch {
   int warm_cycles = 2;  // single-pass cycles while the window settles (LAPH_WARM)
   int strip = 0;  // sweep strip width of the stencil kernels (divides L; L = x-major planes)
   bool strip_fixed = false;  // LAPH_STRIP given
-  int zrun = 1;  // z-sites per wavefront in the FP32 128-vector filter step (LAPH_ZRUN)
+  int zrun = 0;  // z-sites per wavefront of the y-row work-group filter step (LAPH_ZRUN; 0: one site each)
   bool tuned = false;        // filter layout chosen (first solve)
   int poison_ts = -1;  // LAPH_TEST_POISON: NaN into one row of this timeslice's active block at cycle 2 (tests)
   int qr_cycles = 0;  // cycles (and the start block) orthonormalised by explicit CholQR2 (LAPH_QR_CYCLES)

@@ -15,8 +15,10 @@ namespace laph {
 // Filter vectors stored as FP32 complex (bf16 = 0; bw = 64: one vector per
 // lane, 128: two per lane) or as BF16 complex (bf16 = 1; bw = 128 or 256:
 // two or four per lane); buffers block-major with bw-vector blocks.
-// zrun = 2, 4 or 8 (FP32, bw = 128, zrun dividing L): each wavefront walks
-// zrun sites of a z-line, reusing its z-neighbour rows; 1 = one site each.
+// zrun = 1, 2, 4 or 8 (FP32, bw = 128, zrun dividing L, W a multiple of 4):
+// the 4 wavefronts of a work-group take 4 neighbouring y-rows and walk zrun
+// z-sites each, sharing rows through LDS and registers; 0 = one site per
+// wavefront.
 hipError_t launch_filter_step(const float2* U32, const void* in, const void* r, const void* old, void* out,
                               const float4* coef, const float2* cab, const int* first_blk, int L, int K, int nts,
                               int bw, int bf16, int W, int zrun, hipStream_t st);

@@ -297,14 +297,45 @@ __global__ __launch_bounds__(256) void k_filter_step32x2(const float2* __restric
   }
 }
 
-// The 128-vector-block step with each wavefront walking NZ consecutive sites
-// of one z-line (NZ divides L; the sweep index runs z fastest, so a run never
-// leaves its line).  The z-1, z and z+1 rows of the lane's vectors are kept in
-// registers and shifted along the run: a site loads its z+1 row and its four
-// x- and y-neighbour rows, 15 instead of 21 row loads.  Same arithmetic, in
-// the same order, as k_filter_step32x2.
-template <bool HAS_R, int NZ>
-__global__ __launch_bounds__(256) void k_filter_step32x2z(const float2* __restrict__ U32,
+// Link terms of one direction for the lane's two vectors: acc += U_f xu + U_b^H xm.
+__device__ __forceinline__ void link_terms2(const float2* __restrict__ Uf, const float2* __restrict__ Ub,
+                                            const float4 xu[3], const float4 xm[3], float2 acc0[3],
+                                            float2 acc1[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+#pragma unroll
+    for (int bb = 0; bb < 3; ++bb) {
+      const float2 uf = Uf[3 * a + bb], ub = Ub[3 * bb + a];
+      acc0[a] = cfma(uf, make_float2(xu[bb].x, xu[bb].y), acc0[a]);
+      acc1[a] = cfma(uf, make_float2(xu[bb].z, xu[bb].w), acc1[a]);
+      acc0[a] = cfmac(ub, make_float2(xm[bb].x, xm[bb].y), acc0[a]);
+      acc1[a] = cfmac(ub, make_float2(xm[bb].z, xm[bb].w), acc1[a]);
+    }
+  }
+}
+
+// Row c of site `site` of a block-major 128-vector buffer (3 rows of 64
+// float4 per site): a wave-uniform pointer when site is, so the address is
+// formed in scalar registers and the lane index is a 32-bit offset.
+__device__ __forceinline__ const float4* row(const float4* p, int site, int c) {
+  return p + (unsigned)((3 * site + c) * 64);
+}
+
+// The 128-vector-block step with a work-group of 4 wavefronts on 4
+// neighbouring y-rows of one x-plane of a sweep strip (the strip width W is a
+// multiple of 4), each wavefront walking the same NZ z-sites of its row (NZ
+// divides L, so a run never leaves its line).  The z-1, z and z+1 rows of the
+// lane's vectors are kept in registers and shifted along the run; each site's
+// centre rows go through LDS, so the y-neighbours of the inner rows come from
+// the neighbouring wavefronts and only the outer rows of the group load them:
+// 10.5 instead of 21 global row loads per site.  One work-group barrier per
+// site (LDS double-buffered by site parity).  Lanes of a locked 64-vector half
+// (only in the one block per timeslice that straddles the lock boundary) load
+// and compute like the others and only skip their stores, so the wavefronts
+// never diverge.  Site indices are wave-uniform: row addresses and links are
+// formed in scalar registers.
+template <bool HAS_R, bool OLD, int NZ>
+__global__ __launch_bounds__(256) void k_filter_step32x2y(const float2* __restrict__ U32,
                                                            const float4* __restrict__ in,
                                                            const float4* __restrict__ r,
                                                            const float4* __restrict__ old,
@@ -312,37 +343,43 @@ __global__ __launch_bounds__(256) void k_filter_step32x2z(const float2* __restri
                                                            const float4* __restrict__ coef,
                                                            const float2* __restrict__ cab,
                                                            const int* __restrict__ first_blk, int L,
-                                                           int K, int nts, int use_old, int W) {
+                                                           int K, int nts, int W) {
+  __shared__ float4 rows[2][4][3][64];
   const int V3 = L * L * L;
-  const int nrun = V3 / NZ;
-  int ts, cb, sb;
-  unit_map((nrun + 3) / 4, K / 128, nts, ts, cb, sb);
-  if (2 * cb + 1 < first_blk[ts]) return;  // both 64-vector halves inactive
+  const int nzr = L / NZ, nyb = W / 4;
+  int ts, cb, g;
+  unit_map(V3 / (4 * NZ), K / 128, nts, ts, cb, g);
+  const int fbl = first_blk[ts];
+  if (2 * cb + 1 < fbl) return;  // whole work-group: both 64-vector halves inactive
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int ri = sb * 4 + wave;
-  if (ri >= nrun) return;
-  const int s0 = strip_site(ri * NZ, L, W);
+  const int zr = g % nzr;
+  int t = g / nzr;
+  const int yb = t % nyb;
+  t /= nyb;
+  const int x = t % L, strip = t / L;
+  const int y = strip * W + yb * 4 + wave;
+  const int z0 = zr * NZ;
   const int l64 = threadIdx.x & 63;
   const int col = cb * 128 + 2 * l64;
-  if (col < first_blk[ts] * 64) return;  // lanes of a locked 64-vector half issue no traffic
+  const bool act = col >= fbl * 64;
   const size_t vo = ((size_t)ts * (K / 128) + cb) * (size_t)V3 * 3 * 64;  // in float4 units
   const float2* Ut = U32 + (size_t)ts * V3 * 27;
   const int L2 = L * L;
-  const int z0 = s0 % L;
-  const int line = s0 - z0;  // site z = 0 of the run's line
-  const int x = line / L2, y = (line / L) % L;
-  // z = 0 sites of the x- and y-neighbour lines
-  const int lup[2] = {x == L - 1 ? line - (L - 1) * L2 : line + L2, y == L - 1 ? line - (L - 1) * L : line + L};
-  const int ldn[2] = {x == 0 ? line + (L - 1) * L2 : line - L2, y == 0 ? line + (L - 1) * L : line - L};
+  const int line = (x * L + y) * L;  // site z = 0 of the wavefront's line
+  const int xup = x == L - 1 ? line - (L - 1) * L2 : line + L2;
+  const int xdn = x == 0 ? line + (L - 1) * L2 : line - L2;
+  const int yup = y == L - 1 ? line - (L - 1) * L : line + L;
+  const int ydn = y == 0 ? line + (L - 1) * L : line - L;
   const float4* v = in + vo;
   const float2 ca = cab[ts];
+  const float4 cf0 = coef[(size_t)ts * K + col], cf1 = coef[(size_t)ts * K + col + 1];
   float4 zm[3], zc[3];
   {
     const int zp = z0 == 0 ? L - 1 : z0 - 1;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      zm[c] = v[(size_t)(3 * (line + zp) + c) * 64 + l64];
-      zc[c] = v[(size_t)(3 * s0 + c) * 64 + l64];
+      zm[c] = row(v, line + zp, c)[l64];
+      zc[c] = row(v, line + z0, c)[l64];
     }
   }
 #pragma unroll
@@ -351,78 +388,72 @@ __global__ __launch_bounds__(256) void k_filter_step32x2z(const float2* __restri
     const int s = line + z;
     const int zdn = line + (z == 0 ? L - 1 : z - 1);
     const int zup = line + (z == L - 1 ? 0 : z + 1);
-    float4 zn[3];
+    float4 zn[3], yu[3], ym[3], xu[3], xm[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) zn[c] = v[(size_t)(3 * zup + c) * 64 + l64];
+    for (int c = 0; c < 3; ++c) rows[k & 1][wave][c][l64] = zc[c];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      zn[c] = row(v, zup, c)[l64];
+      xu[c] = row(v, xup + z, c)[l64];
+      xm[c] = row(v, xdn + z, c)[l64];
+    }
+    if (wave == 3) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) yu[c] = row(v, yup + z, c)[l64];
+    }
+    if (wave == 0) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ym[c] = row(v, ydn + z, c)[l64];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (wave < 3) yu[c] = rows[k & 1][wave + 1][c][l64];
+      if (wave > 0) ym[c] = rows[k & 1][wave - 1][c][l64];
+    }
     float2 acc0[3], acc1[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       acc0[c] = make_float2(0.f, 0.f);
       acc1[c] = make_float2(0.f, 0.f);
     }
-#pragma unroll
-    for (int mu = 0; mu < 3; ++mu) {
-      float4 xu[3], xm[3];
-      int dn;
-      if (mu < 2) {
-        dn = ldn[mu] + z;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          xu[c] = v[(size_t)(3 * (lup[mu] + z) + c) * 64 + l64];
-          xm[c] = v[(size_t)(3 * dn + c) * 64 + l64];
-        }
-      } else {
-        dn = zdn;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          xu[c] = zn[c];
-          xm[c] = zm[c];
-        }
-      }
-      const float2* Uf = Ut + ((size_t)s * 3 + mu) * 9;
-      const float2* Ub = Ut + ((size_t)dn * 3 + mu) * 9;
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-#pragma unroll
-        for (int bb = 0; bb < 3; ++bb) {
-          const float2 uf = Uf[3 * a + bb], ub = Ub[3 * bb + a];
-          acc0[a] = cfma(uf, make_float2(xu[bb].x, xu[bb].y), acc0[a]);
-          acc1[a] = cfma(uf, make_float2(xu[bb].z, xu[bb].w), acc1[a]);
-          acc0[a] = cfmac(ub, make_float2(xm[bb].x, xm[bb].y), acc0[a]);
-          acc1[a] = cfmac(ub, make_float2(xm[bb].z, xm[bb].w), acc1[a]);
-        }
-      }
-    }
-    const float4 cf0 = coef[(size_t)ts * K + col], cf1 = coef[(size_t)ts * K + col + 1];
+    const int su = __builtin_amdgcn_readfirstlane(s), xd = __builtin_amdgcn_readfirstlane(xdn + z);
+    const int yd = __builtin_amdgcn_readfirstlane(ydn + z), zd = __builtin_amdgcn_readfirstlane(zdn);
+    link_terms2(Ut + ((size_t)su * 3 + 0) * 9, Ut + ((size_t)xd * 3 + 0) * 9, xu, xm, acc0, acc1);
+    link_terms2(Ut + ((size_t)su * 3 + 1) * 9, Ut + ((size_t)yd * 3 + 1) * 9, yu, ym, acc0, acc1);
+    link_terms2(Ut + ((size_t)su * 3 + 2) * 9, Ut + ((size_t)zd * 3 + 2) * 9, zn, zm, acc0, acc1);
+    float4 o[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const size_t idx = vo + (size_t)(3 * s + c) * 64 + l64;
+      const unsigned ro = (unsigned)((3 * su + c) * 64);
       const float4 ctr = zc[c];
-      float4 o;
-      o.x = cf0.x * (ca.x * acc0[c].x + ca.y * ctr.x) + cf0.w * ctr.x;
-      o.y = cf0.x * (ca.x * acc0[c].y + ca.y * ctr.y) + cf0.w * ctr.y;
-      o.z = cf1.x * (ca.x * acc1[c].x + ca.y * ctr.z) + cf1.w * ctr.z;
-      o.w = cf1.x * (ca.x * acc1[c].y + ca.y * ctr.w) + cf1.w * ctr.w;
+      o[c].x = cf0.x * (ca.x * acc0[c].x + ca.y * ctr.x) + cf0.w * ctr.x;
+      o[c].y = cf0.x * (ca.x * acc0[c].y + ca.y * ctr.y) + cf0.w * ctr.y;
+      o[c].z = cf1.x * (ca.x * acc1[c].x + ca.y * ctr.z) + cf1.w * ctr.z;
+      o[c].w = cf1.x * (ca.x * acc1[c].y + ca.y * ctr.w) + cf1.w * ctr.w;
       if (HAS_R) {
-        const float4 rv = ld_nt4(r + idx);
-        o.x += cf0.y * rv.x;
-        o.y += cf0.y * rv.y;
-        o.z += cf1.y * rv.z;
-        o.w += cf1.y * rv.w;
+        const float4 rv = ld_nt4(r + vo + ro + l64);
+        o[c].x += cf0.y * rv.x;
+        o[c].y += cf0.y * rv.y;
+        o[c].z += cf1.y * rv.z;
+        o[c].w += cf1.y * rv.w;
       }
-      if (use_old) {
-        const float4 ov = ld_nt4(old + idx);
-        o.x += cf0.z * ov.x;
-        o.y += cf0.z * ov.y;
-        o.z += cf1.z * ov.z;
-        o.w += cf1.z * ov.w;
+      if (OLD) {
+        const float4 ov = ld_nt4(old + vo + ro + l64);
+        o[c].x += cf0.z * ov.x;
+        o[c].y += cf0.z * ov.y;
+        o[c].z += cf1.z * ov.z;
+        o[c].w += cf1.z * ov.w;
       } else {  // imaginary part of a complex shift
-        o.x -= cf0.z * ctr.y;
-        o.y += cf0.z * ctr.x;
-        o.z -= cf1.z * ctr.w;
-        o.w += cf1.z * ctr.z;
+        o[c].x -= cf0.z * ctr.y;
+        o[c].y += cf0.z * ctr.x;
+        o[c].z -= cf1.z * ctr.w;
+        o[c].w += cf1.z * ctr.z;
       }
-      st_nt4(out + idx, o);
+    }
+    if (act) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) st_nt4(out + vo + (unsigned)((3 * su + c) * 64) + l64, o[c]);
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -765,26 +796,33 @@ hipError_t launch_filter_step(const float2* U32, const void* in_, const void* r_
     const float4* r4 = reinterpret_cast<const float4*>(r);
     const float4* o4 = reinterpret_cast<const float4*>(old);
     float4* out4 = reinterpret_cast<float4*>(out);
-    if (zrun > 1) {
-      if (L % zrun != 0) return hipErrorInvalidValue;
-      const dim3 gz((unsigned)((V3 / zrun + 3) / 4) * (unsigned)(K / 128) * (unsigned)nts);
-#define LAPH_ZRUN_CASE(NZ)                                                                                    \
-  case NZ:                                                                                                    \
-    if (r)                                                                                                    \
-      k_filter_step32x2z<true, NZ><<<gz, 256, 0, st>>>(U32, in4, r4, o4, out4, coef, cab, first_blk, L, K, nts, \
-                                                       old != nullptr, W);                                    \
-    else                                                                                                      \
-      k_filter_step32x2z<false, NZ><<<gz, 256, 0, st>>>(U32, in4, r4, o4, out4, coef, cab, first_blk, L, K,    \
-                                                        nts, old != nullptr, W);                              \
+    if (zrun > 0) {
+      if (L % zrun != 0 || W % 4 != 0) return hipErrorInvalidValue;
+      const dim3 gy((unsigned)(V3 / (4 * zrun)) * (unsigned)(K / 128) * (unsigned)nts);
+#define LAPH_YGROUP_CASE(NZ)                                                                                   \
+  case NZ:                                                                                                     \
+    if (r && old)                                                                                              \
+      k_filter_step32x2y<true, true, NZ><<<gy, 256, 0, st>>>(U32, in4, r4, o4, out4, coef, cab, first_blk, L, K, \
+                                                             nts, W);                                          \
+    else if (r)                                                                                                \
+      k_filter_step32x2y<true, false, NZ><<<gy, 256, 0, st>>>(U32, in4, r4, o4, out4, coef, cab, first_blk, L,   \
+                                                              K, nts, W);                                      \
+    else if (old)                                                                                              \
+      k_filter_step32x2y<false, true, NZ><<<gy, 256, 0, st>>>(U32, in4, r4, o4, out4, coef, cab, first_blk, L,   \
+                                                              K, nts, W);                                      \
+    else                                                                                                       \
+      k_filter_step32x2y<false, false, NZ><<<gy, 256, 0, st>>>(U32, in4, r4, o4, out4, coef, cab, first_blk, L,  \
+                                                               K, nts, W);                                     \
     return hipGetLastError();
       switch (zrun) {
-        LAPH_ZRUN_CASE(2)
-        LAPH_ZRUN_CASE(4)
-        LAPH_ZRUN_CASE(8)
+        LAPH_YGROUP_CASE(1)
+        LAPH_YGROUP_CASE(2)
+        LAPH_YGROUP_CASE(4)
+        LAPH_YGROUP_CASE(8)
         default:
           return hipErrorInvalidValue;
       }
-#undef LAPH_ZRUN_CASE
+#undef LAPH_YGROUP_CASE
     }
     if (r)
       k_filter_step32x2<true><<<g, 256, 0, st>>>(U32, in4, r4, o4, out4, coef, cab, first_blk, L, K, nts,

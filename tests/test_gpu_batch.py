@@ -121,12 +121,14 @@ def test_filter_block_widths(lp, gpu, storage, width, monkeypatch):
     b.close()
 
 
-@pytest.mark.parametrize("zrun", [1, 2, 4, 8])
+@pytest.mark.parametrize("zrun", [0, 1, 2, 4, 8])
 def test_filter_z_runs(lp, gpu, zrun, monkeypatch):
-    """The FP32 128-vector filter step with each wavefront walking zrun sites
-    of a z-line (z-neighbour rows shifted through registers; the run's first
-    and last sites take their outer z-neighbour across the periodic edge at
-    L = 8) gives the dense oracle's spectrum, in both strip orders."""
+    """The FP32 128-vector filter step in work-groups of 4 y-rows, each
+    wavefront walking zrun sites of its z-line (z-neighbour rows shifted
+    through registers, y-neighbours of the inner rows through LDS; at L = 8
+    the runs and the groups meet the periodic edges in z and y), and the
+    one-site-per-wavefront step (zrun 0), give the dense oracle's spectrum in
+    both strip orders."""
     L, nts, nev, deg, tol = 8, 2, 40, 8, 1e-10
     data = smeared(L, 57, 0.4, nts)
     monkeypatch.setenv("LAPH_BW", "128")
@@ -145,7 +147,7 @@ def test_filter_z_runs(lp, gpu, zrun, monkeypatch):
 
 
 def test_filter_z_run_rejected(lp, gpu, monkeypatch):
-    """A z-run that does not divide L is refused at creation."""
+    """A z-run on a lattice whose L is not a multiple of 4 is refused at creation."""
     monkeypatch.setenv("LAPH_ZRUN", "4")
     with pytest.raises(lp.LaphError):
         lp.Batch(6, 8, 12.0, 0.0, 1, 10, nsub=64, tol=1e-10, seed=1, device=gpu)
