@@ -422,7 +422,7 @@ static int rayleigh_ritz(const View& v, Act q, bool orthonormal, bool* chol_fail
   // residuals of the new Ritz pairs for the next filter pass (D X is not kept:
   // the next Rayleigh-Ritz applies D to the filtered block)
   LAPH_HIP(launch_lap_residual64(b->U64 + v.lo(), T, b->mu + v.nk(), b->p.lambda_l, fb(b, b->R32, v.vo()), b->p.L,
-                                 K, nt, b->lock_blk + v.ln->t0, b->strip, b->bw, b->bf16, v.ln->st));
+                                 K, nt, b->lock_blk + v.ln->t0, b->strip, b->bw, b->bf16, stencil_zrun(b), v.ln->st));
   // the hard-locked columns of X live in the old X buffer: carry them over
   if (q.l > 0)
     LAPH_HIP(hipMemcpy2DAsync(T, (size_t)K * sizeof(double2), Y, (size_t)K * sizeof(double2),
@@ -455,7 +455,7 @@ static int orthonormalise_and_rr(const View& v, Act q, bool explicit_qr) {
       LAPH_TRY(cholqr(v, q));
     }
     LAPH_HIP(launch_lap_block64(b->U64 + v.lo(), v.X(), v.AX(), b->p.L, b->K, v.ln->nt, b->lock_blk + v.ln->t0,
-                                b->strip, v.ln->st));
+                                b->strip, stencil_zrun(b), v.ln->st));
     v.ln->sv64 += 2LL * v.ln->nt * q.a;  // D Y for Rayleigh-Ritz, D X after it
     bool failed = false;
     LAPH_TRY(rayleigh_ritz(v, q, explicit_qr || pass == 1, &failed));
@@ -477,7 +477,8 @@ static int start_from_block(const View& v) {
   LAPH_HIP(hipMemcpyAsync(b->mu + v.nk(), b->h_mu.data() + v.nk(), nk * sizeof(double), hipMemcpyHostToDevice,
                           v.ln->st));
   LAPH_HIP(launch_lap_residual64(b->U64 + v.lo(), v.X(), b->mu + v.nk(), b->p.lambda_l, fb(b, b->R32, v.vo()),
-                                 b->p.L, b->K, v.ln->nt, b->lock_blk + v.ln->t0, b->strip, b->bw, b->bf16, v.ln->st));
+                                 b->p.L, b->K, v.ln->nt, b->lock_blk + v.ln->t0, b->strip, b->bw, b->bf16,
+                                 stencil_zrun(b), v.ln->st));
   v.ln->sv64 += (long long)v.ln->nt * b->K;
   return LAPH_OK;
 }
@@ -741,7 +742,7 @@ static int enqueue_filter(const View& v) {
     void* out = fb(b, b->P[(st - 1) % nb], vo);
     LAPH_HIP(launch_filter_step(b->U32 + v.lo(), in, b->qform ? nullptr : R, old, out,
                                 b->coef + (st - 1) * step + v.nk(),
-                                b->cab + v.ln->t0, b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, b->strip, b->strip % 4 == 0 ? b->zrun : 0, v.ln->st));
+                                b->cab + v.ln->t0, b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, b->strip, stencil_zrun(b), v.ln->st));
   }
   return LAPH_OK;
 }
@@ -816,7 +817,7 @@ static int enqueue_plain_pass(const View& v, int p) {
     const bool last = st == m;
     const float4* co = b->coef2 + (size_t)(last ? m - 1 + p - 2 : st - 1) * step + v.nk();
     LAPH_HIP(launch_filter_step(b->U32 + v.lo(), in, last ? f : nullptr, old, out, co, b->cab + v.ln->t0,
-                                b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, b->strip, b->strip % 4 == 0 ? b->zrun : 0, v.ln->st));
+                                b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, b->strip, stencil_zrun(b), v.ln->st));
   }
   return LAPH_OK;
 }
@@ -1155,7 +1156,7 @@ static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
   s.iterations = cycles;
   s.filter_block = b->bw;
   s.filter_strip = b->strip;
-  s.filter_zrun = b->bf16 == 0 && b->bw == 128 && b->strip % 4 == 0 ? b->zrun : 0;
+  s.filter_zrun = b->bf16 == 0 && b->bw == 128 ? laph::stencil_zrun(b) : 0;
   s.converged = conv;
   s.max_residual = maxres;
   s.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -122,6 +122,9 @@ struct laph_batch {
 };
 
 namespace laph {
+// z-sites per wavefront of the y-row work-group stencils for the batch's sweep
+// strip (they need a strip width that is a multiple of 4); 0: one site each.
+inline int stencil_zrun(const laph_batch* b) { return b->strip % 4 == 0 ? b->zrun : 0; }
 // Allocates laph_batch::wV / wph on first use.
 int output_buffers(laph_batch* b);
 }  // namespace laph

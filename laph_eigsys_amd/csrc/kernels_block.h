@@ -24,14 +24,16 @@ hipError_t launch_filter_step(const float2* U32, const void* in, const void* r, 
                               int bw, int bf16, int W, int zrun, hipStream_t st);
 // out = D v on every (or, with first_blk, every active) column.
 // W: sweep strip width (divides L; L = plain x-major order)
+// zrun > 0 (dividing L, W a multiple of 4): work-groups of 4 y-rows, zrun
+// z-sites per wavefront, as the filter step; 0: one site per wavefront.
 hipError_t launch_lap_block64(const double2* U64, const double2* v, double2* out, int L, int K, int nts,
-                              const int* first_blk, int W, hipStream_t st);
+                              const int* first_blk, int W, int zrun, hipStream_t st);
 // Residuals of the Ritz pairs from the FP64 stencil: r = c_t (D X + X diag(mu))
 // in the filter storage (block-major, bw-vector blocks, FP32 or BF16), mu the
 // Ritz values of -D ([nts][K]), c_t = 2 / (lamL - min(mu[t][K-1], 0.75 lamL)),
 // for the 64-column blocks >= first_blk[ts].
 hipError_t launch_lap_residual64(const double2* U64, const double2* X, const double* mu, double lamL, void* r,
-                                 int L, int K, int nts, const int* first_blk, int W, int bw, int bf16,
+                                 int L, int K, int nts, const int* first_blk, int W, int bw, int bf16, int zrun,
                                  hipStream_t st);
 // nrm2 = column sums of |D X + X diag(mu)|^2 from that r, for the columns
 // >= first_col (a multiple of 64).

@@ -672,6 +672,139 @@ __global__ __launch_bounds__(256) void k_lap_residual64(const double2* __restric
   }
 }
 
+// The FP64 stencils on 64-column blocks in work-groups of 4 neighbouring
+// y-rows of one x-plane of a sweep strip (W a multiple of 4), each wavefront
+// walking NZ z-sites of its row: z-neighbour rows shifted through registers,
+// y-neighbours of the inner rows read from the neighbouring wavefronts' LDS
+// copy of their centre rows (one barrier per site, LDS double-buffered by
+// site parity), row addresses formed in scalar registers.  RES = false:
+// out = D v in the site-major layout; RES = true: the Ritz residuals
+// r = c_t (D v + v diag(mu)) in the filter storage (BW-vector blocks, FP32
+// or BF16), as k_lap_residual64.
+template <bool RES, int BW, bool BF16, int NZ>
+__global__ __launch_bounds__(256) void k_lap64y(const double2* __restrict__ U64, const double2* __restrict__ v,
+                                                double2* __restrict__ out, const double* __restrict__ mu,
+                                                double lamL, void* __restrict__ rbuf, int L, int K, int nts,
+                                                const int* __restrict__ first_blk, int W) {
+  __shared__ double2 rows[2][4][3][64];
+  const int V3 = L * L * L;
+  const int nzr = L / NZ, nyb = W / 4;
+  int ts, cb, g;
+  unit_map(V3 / (4 * NZ), K / 64, nts, ts, cb, g);
+  if (first_blk && cb < first_blk[ts]) return;  // whole work-group
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int zr = g % nzr;
+  int t = g / nzr;
+  const int yb = t % nyb;
+  t /= nyb;
+  const int x = t % L, strip = t / L;
+  const int y = strip * W + yb * 4 + wave;
+  const int z0 = zr * NZ;
+  const int l64 = threadIdx.x & 63;
+  const double2* vb = v + (size_t)ts * V3 * 3 * K + cb * 64;  // column block of the timeslice
+  const double2* Ut = U64 + (size_t)ts * V3 * 27;
+  const int L2 = L * L;
+  const int line = (x * L + y) * L;
+  const int xup = x == L - 1 ? line - (L - 1) * L2 : line + L2;
+  const int xdn = x == 0 ? line + (L - 1) * L2 : line - L2;
+  const int yup = y == L - 1 ? line - (L - 1) * L : line + L;
+  const int ydn = y == 0 ? line + (L - 1) * L : line - L;
+  auto rw = [&](int site, int c) { return vb + (unsigned)((3 * site + c) * K); };
+  double cs = 0.0, mc = 0.0;
+  if (RES) {
+    const double* m = mu + (size_t)ts * K;
+    cs = window_scale(m, K, lamL);
+    mc = m[cb * 64 + l64] - 6.0;
+  }
+  double2 zm[3], zc[3];
+  {
+    const int zp = z0 == 0 ? L - 1 : z0 - 1;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      zm[c] = rw(line + zp, c)[l64];
+      zc[c] = rw(line + z0, c)[l64];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NZ; ++k) {
+    const int z = z0 + k;
+    const int s = line + z;
+    const int zdn = line + (z == 0 ? L - 1 : z - 1);
+    const int zup = line + (z == L - 1 ? 0 : z + 1);
+    double2 zn[3], yu[3], ym[3], xu[3], xm[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) rows[k & 1][wave][c][l64] = zc[c];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      zn[c] = rw(zup, c)[l64];
+      xu[c] = rw(xup + z, c)[l64];
+      xm[c] = rw(xdn + z, c)[l64];
+    }
+    if (wave == 3) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) yu[c] = rw(yup + z, c)[l64];
+    }
+    if (wave == 0) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ym[c] = rw(ydn + z, c)[l64];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (wave < 3) yu[c] = rows[k & 1][wave + 1][c][l64];
+      if (wave > 0) ym[c] = rows[k & 1][wave - 1][c][l64];
+    }
+    const int su = __builtin_amdgcn_readfirstlane(s);
+    const int dn[3] = {__builtin_amdgcn_readfirstlane(xdn + z), __builtin_amdgcn_readfirstlane(ydn + z),
+                       __builtin_amdgcn_readfirstlane(zdn)};
+    double2 acc[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[c] = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int mu_ = 0; mu_ < 3; ++mu_) {
+      const double2* nu = mu_ == 0 ? xu : (mu_ == 1 ? yu : zn);
+      const double2* nd = mu_ == 0 ? xm : (mu_ == 1 ? ym : zm);
+      const double2* Uf = Ut + ((size_t)su * 3 + mu_) * 9;
+      const double2* Ub = Ut + ((size_t)dn[mu_] * 3 + mu_) * 9;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb) {
+          acc[a] = cfma(Uf[3 * a + bb], nu[bb], acc[a]);
+          acc[a] = cfmac(Ub[3 * bb + a], nd[bb], acc[a]);
+        }
+      }
+    }
+    if (!RES) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double2 o;
+        o.x = acc[c].x - 6.0 * zc[c].x;
+        o.y = acc[c].y - 6.0 * zc[c].y;
+        st_nt(out + (size_t)ts * V3 * 3 * K + cb * 64 + (unsigned)((3 * su + c) * K) + l64, o);
+      }
+    } else {
+      // block-major filter storage: block cb*64/BW, offset (cb*64) % BW + l64 within its rows
+      const size_t n = (size_t)3 * V3;
+      const size_t rbase = ((size_t)ts * (K / BW) + (cb * 64) / BW) * n * BW + (cb * 64) % BW;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const float2 rv = make_float2((float)(cs * (acc[a].x + mc * zc[a].x)), (float)(cs * (acc[a].y + mc * zc[a].y)));
+        const size_t ri = rbase + (unsigned)((3 * su + a) * BW) + l64;
+        if (BF16)
+          static_cast<unsigned*>(rbuf)[ri] = bf2_pack(rv);
+        else
+          static_cast<float2*>(rbuf)[ri] = rv;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      zm[c] = zc[c];
+      zc[c] = zn[c];
+    }
+  }
+}
+
 // nrm2[ts][col] += |R|^2 summed over the rows of one chunk, from r = c_t R in
 // the filter storage, for the columns >= first_col.  grid (row chunks,
 // active 64-column groups, ts); each thread sums the rows of one column.
@@ -841,15 +974,54 @@ hipError_t launch_filter_step(const float2* U32, const void* in_, const void* r_
   return hipGetLastError();
 }
 
+template <bool RES, int BW, bool BF16>
+static hipError_t launch_lap64y(const double2* U64, const double2* v, double2* out, const double* mu, double lamL,
+                                void* r, int L, int K, int nts, const int* first_blk, int W, int zrun, hipStream_t st) {
+  const dim3 g((unsigned)(L * L * L / (4 * zrun)) * (unsigned)(K / 64) * (unsigned)nts);
+  switch (zrun) {
+    case 1:
+      k_lap64y<RES, BW, BF16, 1><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W);
+      break;
+    case 2:
+      k_lap64y<RES, BW, BF16, 2><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W);
+      break;
+    case 4:
+      k_lap64y<RES, BW, BF16, 4><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W);
+      break;
+    case 8:
+      k_lap64y<RES, BW, BF16, 8><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_lap_block64(const double2* U64, const double2* v, double2* out, int L, int K, int nts,
-                              const int* first_blk, int W, hipStream_t st) {
+                              const int* first_blk, int W, int zrun, hipStream_t st) {
+  if (zrun > 0) {
+    if (L % zrun != 0 || W % 4 != 0) return hipErrorInvalidValue;
+    return launch_lap64y<false, 64, false>(U64, v, out, nullptr, 0.0, nullptr, L, K, nts, first_blk, W, zrun, st);
+  }
   k_lap_block64<<<site_grid(L, K, nts), 256, 0, st>>>(U64, v, out, L, K, nts, first_blk, W);
   return hipGetLastError();
 }
 
 hipError_t launch_lap_residual64(const double2* U64, const double2* X, const double* mu, double lamL, void* r,
-                                 int L, int K, int nts, const int* first_blk, int W, int bw, int bf16,
+                                 int L, int K, int nts, const int* first_blk, int W, int bw, int bf16, int zrun,
                                  hipStream_t st) {
+  if (zrun > 0) {
+    if (L % zrun != 0 || W % 4 != 0) return hipErrorInvalidValue;
+    if (bf16 && bw == 256)
+      return launch_lap64y<true, 256, true>(U64, X, nullptr, mu, lamL, r, L, K, nts, first_blk, W, zrun, st);
+    if (bf16 && bw == 128)
+      return launch_lap64y<true, 128, true>(U64, X, nullptr, mu, lamL, r, L, K, nts, first_blk, W, zrun, st);
+    if (!bf16 && bw == 128)
+      return launch_lap64y<true, 128, false>(U64, X, nullptr, mu, lamL, r, L, K, nts, first_blk, W, zrun, st);
+    if (!bf16 && bw == 64)
+      return launch_lap64y<true, 64, false>(U64, X, nullptr, mu, lamL, r, L, K, nts, first_blk, W, zrun, st);
+    return hipErrorInvalidValue;
+  }
   k_lap_residual64<<<site_grid(L, K, nts), 256, 0, st>>>(U64, X, mu, lamL, r, L, K, nts, first_blk, W, bw, bf16);
   return hipGetLastError();
 }
