@@ -323,12 +323,14 @@ __device__ __forceinline__ const float4* row(const float4* p, int site, int c) {
 
 // The 128-vector-block step with a work-group of 4 wavefronts on 4
 // neighbouring y-rows of one x-plane of a sweep strip (the strip width W is a
-// multiple of 4), each wavefront walking the same NZ z-sites of its row (NZ
-// divides L, so a run never leaves its line).  The z-1, z and z+1 rows of the
-// lane's vectors are kept in registers and shifted along the run; each site's
+// multiple of 4), each wavefront walking the whole z-line of its row in
+// unrolled runs of NZ sites (NZ divides L).  The z-1, z and z+1 rows of the
+// lane's vectors are kept in registers and shifted along the line; each site's
 // centre rows go through LDS, so the y-neighbours of the inner rows come from
 // the neighbouring wavefronts and only the outer rows of the group load them:
-// 10.5 instead of 21 global row loads per site.  One work-group barrier per
+// 10.5 instead of 21 global row loads per site.  The work-groups of one unit
+// run y-block, then x, then strip, so the lines in flight on an XCD are the
+// neighbouring x-planes of one strip, all advancing along z.  One work-group barrier per
 // site (LDS double-buffered by site parity).  Lanes of a locked 64-vector half
 // (only in the one block per timeslice that straddles the lock boundary) load
 // and compute like the others and only skip their stores, so the wavefronts
@@ -346,19 +348,16 @@ __global__ __launch_bounds__(256) void k_filter_step32x2y(const float2* __restri
                                                            int K, int nts, int W) {
   __shared__ float4 rows[2][4][3][64];
   const int V3 = L * L * L;
-  const int nzr = L / NZ, nyb = W / 4;
+  const int nyb = W / 4;
   int ts, cb, g;
-  unit_map(V3 / (4 * NZ), K / 128, nts, ts, cb, g);
+  unit_map(V3 / (4 * L), K / 128, nts, ts, cb, g);
   const int fbl = first_blk[ts];
   if (2 * cb + 1 < fbl) return;  // whole work-group: both 64-vector halves inactive
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int zr = g % nzr;
-  int t = g / nzr;
-  const int yb = t % nyb;
-  t /= nyb;
+  const int yb = g % nyb;
+  const int t = g / nyb;
   const int x = t % L, strip = t / L;
   const int y = strip * W + yb * 4 + wave;
-  const int z0 = zr * NZ;
   const int l64 = threadIdx.x & 63;
   const int col = cb * 128 + 2 * l64;
   const bool act = col >= fbl * 64;
@@ -375,90 +374,92 @@ __global__ __launch_bounds__(256) void k_filter_step32x2y(const float2* __restri
   const float4 cf0 = coef[(size_t)ts * K + col], cf1 = coef[(size_t)ts * K + col + 1];
   float4 zm[3], zc[3];
   {
-    const int zp = z0 == 0 ? L - 1 : z0 - 1;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      zm[c] = row(v, line + zp, c)[l64];
-      zc[c] = row(v, line + z0, c)[l64];
+      zm[c] = row(v, line + L - 1, c)[l64];
+      zc[c] = row(v, line, c)[l64];
     }
   }
+  for (int z0 = 0; z0 < L; z0 += NZ) {
 #pragma unroll
-  for (int k = 0; k < NZ; ++k) {
-    const int z = z0 + k;
-    const int s = line + z;
-    const int zdn = line + (z == 0 ? L - 1 : z - 1);
-    const int zup = line + (z == L - 1 ? 0 : z + 1);
-    float4 zn[3], yu[3], ym[3], xu[3], xm[3];
+    for (int k = 0; k < NZ; ++k) {
+      const int z = z0 + k;
+      const int par = z & 1;
+      const int s = line + z;
+      const int zdn = line + (z == 0 ? L - 1 : z - 1);
+      const int zup = line + (z == L - 1 ? 0 : z + 1);
+      float4 zn[3], yu[3], ym[3], xu[3], xm[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) rows[k & 1][wave][c][l64] = zc[c];
+      for (int c = 0; c < 3; ++c) rows[par][wave][c][l64] = zc[c];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      zn[c] = row(v, zup, c)[l64];
-      xu[c] = row(v, xup + z, c)[l64];
-      xm[c] = row(v, xdn + z, c)[l64];
-    }
-    if (wave == 3) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) yu[c] = row(v, yup + z, c)[l64];
-    }
-    if (wave == 0) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) ym[c] = row(v, ydn + z, c)[l64];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      if (wave < 3) yu[c] = rows[k & 1][wave + 1][c][l64];
-      if (wave > 0) ym[c] = rows[k & 1][wave - 1][c][l64];
-    }
-    float2 acc0[3], acc1[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      acc0[c] = make_float2(0.f, 0.f);
-      acc1[c] = make_float2(0.f, 0.f);
-    }
-    const int su = __builtin_amdgcn_readfirstlane(s), xd = __builtin_amdgcn_readfirstlane(xdn + z);
-    const int yd = __builtin_amdgcn_readfirstlane(ydn + z), zd = __builtin_amdgcn_readfirstlane(zdn);
-    link_terms2(Ut + ((size_t)su * 3 + 0) * 9, Ut + ((size_t)xd * 3 + 0) * 9, xu, xm, acc0, acc1);
-    link_terms2(Ut + ((size_t)su * 3 + 1) * 9, Ut + ((size_t)yd * 3 + 1) * 9, yu, ym, acc0, acc1);
-    link_terms2(Ut + ((size_t)su * 3 + 2) * 9, Ut + ((size_t)zd * 3 + 2) * 9, zn, zm, acc0, acc1);
-    float4 o[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const unsigned ro = (unsigned)((3 * su + c) * 64);
-      const float4 ctr = zc[c];
-      o[c].x = cf0.x * (ca.x * acc0[c].x + ca.y * ctr.x) + cf0.w * ctr.x;
-      o[c].y = cf0.x * (ca.x * acc0[c].y + ca.y * ctr.y) + cf0.w * ctr.y;
-      o[c].z = cf1.x * (ca.x * acc1[c].x + ca.y * ctr.z) + cf1.w * ctr.z;
-      o[c].w = cf1.x * (ca.x * acc1[c].y + ca.y * ctr.w) + cf1.w * ctr.w;
-      if (HAS_R) {
-        const float4 rv = ld_nt4(r + vo + ro + l64);
-        o[c].x += cf0.y * rv.x;
-        o[c].y += cf0.y * rv.y;
-        o[c].z += cf1.y * rv.z;
-        o[c].w += cf1.y * rv.w;
+      for (int c = 0; c < 3; ++c) {
+        zn[c] = row(v, zup, c)[l64];
+        xu[c] = row(v, xup + z, c)[l64];
+        xm[c] = row(v, xdn + z, c)[l64];
       }
-      if (OLD) {
-        const float4 ov = ld_nt4(old + vo + ro + l64);
-        o[c].x += cf0.z * ov.x;
-        o[c].y += cf0.z * ov.y;
-        o[c].z += cf1.z * ov.z;
-        o[c].w += cf1.z * ov.w;
-      } else {  // imaginary part of a complex shift
-        o[c].x -= cf0.z * ctr.y;
-        o[c].y += cf0.z * ctr.x;
-        o[c].z -= cf1.z * ctr.w;
-        o[c].w += cf1.z * ctr.z;
+      if (wave == 3) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) yu[c] = row(v, yup + z, c)[l64];
       }
-    }
-    if (act) {
+      if (wave == 0) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) st_nt4(out + vo + (unsigned)((3 * su + c) * 64) + l64, o[c]);
-    }
+        for (int c = 0; c < 3; ++c) ym[c] = row(v, ydn + z, c)[l64];
+      }
+      __syncthreads();
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      zm[c] = zc[c];
-      zc[c] = zn[c];
+      for (int c = 0; c < 3; ++c) {
+        if (wave < 3) yu[c] = rows[par][wave + 1][c][l64];
+        if (wave > 0) ym[c] = rows[par][wave - 1][c][l64];
+      }
+      float2 acc0[3], acc1[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        acc0[c] = make_float2(0.f, 0.f);
+        acc1[c] = make_float2(0.f, 0.f);
+      }
+      const int su = __builtin_amdgcn_readfirstlane(s), xd = __builtin_amdgcn_readfirstlane(xdn + z);
+      const int yd = __builtin_amdgcn_readfirstlane(ydn + z), zd = __builtin_amdgcn_readfirstlane(zdn);
+      link_terms2(Ut + ((size_t)su * 3 + 0) * 9, Ut + ((size_t)xd * 3 + 0) * 9, xu, xm, acc0, acc1);
+      link_terms2(Ut + ((size_t)su * 3 + 1) * 9, Ut + ((size_t)yd * 3 + 1) * 9, yu, ym, acc0, acc1);
+      link_terms2(Ut + ((size_t)su * 3 + 2) * 9, Ut + ((size_t)zd * 3 + 2) * 9, zn, zm, acc0, acc1);
+      float4 o[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const unsigned ro = (unsigned)((3 * su + c) * 64);
+        const float4 ctr = zc[c];
+        o[c].x = cf0.x * (ca.x * acc0[c].x + ca.y * ctr.x) + cf0.w * ctr.x;
+        o[c].y = cf0.x * (ca.x * acc0[c].y + ca.y * ctr.y) + cf0.w * ctr.y;
+        o[c].z = cf1.x * (ca.x * acc1[c].x + ca.y * ctr.z) + cf1.w * ctr.z;
+        o[c].w = cf1.x * (ca.x * acc1[c].y + ca.y * ctr.w) + cf1.w * ctr.w;
+        if (HAS_R) {
+          const float4 rv = ld_nt4(r + vo + ro + l64);
+          o[c].x += cf0.y * rv.x;
+          o[c].y += cf0.y * rv.y;
+          o[c].z += cf1.y * rv.z;
+          o[c].w += cf1.y * rv.w;
+        }
+        if (OLD) {
+          const float4 ov = ld_nt4(old + vo + ro + l64);
+          o[c].x += cf0.z * ov.x;
+          o[c].y += cf0.z * ov.y;
+          o[c].z += cf1.z * ov.z;
+          o[c].w += cf1.z * ov.w;
+        } else {  // imaginary part of a complex shift
+          o[c].x -= cf0.z * ctr.y;
+          o[c].y += cf0.z * ctr.x;
+          o[c].z -= cf1.z * ctr.w;
+          o[c].w += cf1.z * ctr.z;
+        }
+      }
+      if (act) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) st_nt4(out + vo + (unsigned)((3 * su + c) * 64) + l64, o[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        zm[c] = zc[c];
+        zc[c] = zn[c];
+      }
     }
   }
 }
@@ -674,7 +675,8 @@ __global__ __launch_bounds__(256) void k_lap_residual64(const double2* __restric
 
 // The FP64 stencils on 64-column blocks in work-groups of 4 neighbouring
 // y-rows of one x-plane of a sweep strip (W a multiple of 4), each wavefront
-// walking NZ z-sites of its row: z-neighbour rows shifted through registers,
+// walking the z-line of its row in unrolled runs of NZ sites, as the filter
+// step k_filter_step32x2y: z-neighbour rows shifted through registers,
 // y-neighbours of the inner rows read from the neighbouring wavefronts' LDS
 // copy of their centre rows (one barrier per site, LDS double-buffered by
 // site parity), row addresses formed in scalar registers.  RES = false:
@@ -688,18 +690,15 @@ __global__ __launch_bounds__(256) void k_lap64y(const double2* __restrict__ U64,
                                                 const int* __restrict__ first_blk, int W) {
   __shared__ double2 rows[2][4][3][64];
   const int V3 = L * L * L;
-  const int nzr = L / NZ, nyb = W / 4;
+  const int nyb = W / 4;
   int ts, cb, g;
-  unit_map(V3 / (4 * NZ), K / 64, nts, ts, cb, g);
+  unit_map(V3 / (4 * L), K / 64, nts, ts, cb, g);
   if (first_blk && cb < first_blk[ts]) return;  // whole work-group
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int zr = g % nzr;
-  int t = g / nzr;
-  const int yb = t % nyb;
-  t /= nyb;
+  const int yb = g % nyb;
+  const int t = g / nyb;
   const int x = t % L, strip = t / L;
   const int y = strip * W + yb * 4 + wave;
-  const int z0 = zr * NZ;
   const int l64 = threadIdx.x & 63;
   const double2* vb = v + (size_t)ts * V3 * 3 * K + cb * 64;  // column block of the timeslice
   const double2* Ut = U64 + (size_t)ts * V3 * 27;
@@ -718,89 +717,92 @@ __global__ __launch_bounds__(256) void k_lap64y(const double2* __restrict__ U64,
   }
   double2 zm[3], zc[3];
   {
-    const int zp = z0 == 0 ? L - 1 : z0 - 1;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      zm[c] = rw(line + zp, c)[l64];
-      zc[c] = rw(line + z0, c)[l64];
+      zm[c] = rw(line + L - 1, c)[l64];
+      zc[c] = rw(line, c)[l64];
     }
   }
+  for (int z0 = 0; z0 < L; z0 += NZ) {
 #pragma unroll
-  for (int k = 0; k < NZ; ++k) {
-    const int z = z0 + k;
-    const int s = line + z;
-    const int zdn = line + (z == 0 ? L - 1 : z - 1);
-    const int zup = line + (z == L - 1 ? 0 : z + 1);
-    double2 zn[3], yu[3], ym[3], xu[3], xm[3];
+    for (int k = 0; k < NZ; ++k) {
+      const int z = z0 + k;
+      const int par = z & 1;
+      const int s = line + z;
+      const int zdn = line + (z == 0 ? L - 1 : z - 1);
+      const int zup = line + (z == L - 1 ? 0 : z + 1);
+      double2 zn[3], yu[3], ym[3], xu[3], xm[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) rows[k & 1][wave][c][l64] = zc[c];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      zn[c] = rw(zup, c)[l64];
-      xu[c] = rw(xup + z, c)[l64];
-      xm[c] = rw(xdn + z, c)[l64];
-    }
-    if (wave == 3) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) yu[c] = rw(yup + z, c)[l64];
-    }
-    if (wave == 0) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) ym[c] = rw(ydn + z, c)[l64];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      if (wave < 3) yu[c] = rows[k & 1][wave + 1][c][l64];
-      if (wave > 0) ym[c] = rows[k & 1][wave - 1][c][l64];
-    }
-    const int su = __builtin_amdgcn_readfirstlane(s);
-    const int dn[3] = {__builtin_amdgcn_readfirstlane(xdn + z), __builtin_amdgcn_readfirstlane(ydn + z),
-                       __builtin_amdgcn_readfirstlane(zdn)};
-    double2 acc[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) acc[c] = make_double2(0.0, 0.0);
-#pragma unroll
-    for (int mu_ = 0; mu_ < 3; ++mu_) {
-      const double2* nu = mu_ == 0 ? xu : (mu_ == 1 ? yu : zn);
-      const double2* nd = mu_ == 0 ? xm : (mu_ == 1 ? ym : zm);
-      const double2* Uf = Ut + ((size_t)su * 3 + mu_) * 9;
-      const double2* Ub = Ut + ((size_t)dn[mu_] * 3 + mu_) * 9;
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-#pragma unroll
-        for (int bb = 0; bb < 3; ++bb) {
-          acc[a] = cfma(Uf[3 * a + bb], nu[bb], acc[a]);
-          acc[a] = cfmac(Ub[3 * bb + a], nd[bb], acc[a]);
-        }
-      }
-    }
-    if (!RES) {
+      for (int c = 0; c < 3; ++c) rows[par][wave][c][l64] = zc[c];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        double2 o;
-        o.x = acc[c].x - 6.0 * zc[c].x;
-        o.y = acc[c].y - 6.0 * zc[c].y;
-        st_nt(out + (size_t)ts * V3 * 3 * K + cb * 64 + (unsigned)((3 * su + c) * K) + l64, o);
+        zn[c] = rw(zup, c)[l64];
+        xu[c] = rw(xup + z, c)[l64];
+        xm[c] = rw(xdn + z, c)[l64];
       }
-    } else {
-      // block-major filter storage: block cb*64/BW, offset (cb*64) % BW + l64 within its rows
-      const size_t n = (size_t)3 * V3;
-      const size_t rbase = ((size_t)ts * (K / BW) + (cb * 64) / BW) * n * BW + (cb * 64) % BW;
+      if (wave == 3) {
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const float2 rv = make_float2((float)(cs * (acc[a].x + mc * zc[a].x)), (float)(cs * (acc[a].y + mc * zc[a].y)));
-        const size_t ri = rbase + (unsigned)((3 * su + a) * BW) + l64;
-        if (BF16)
-          static_cast<unsigned*>(rbuf)[ri] = bf2_pack(rv);
-        else
-          static_cast<float2*>(rbuf)[ri] = rv;
+        for (int c = 0; c < 3; ++c) yu[c] = rw(yup + z, c)[l64];
       }
-    }
+      if (wave == 0) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      zm[c] = zc[c];
-      zc[c] = zn[c];
+        for (int c = 0; c < 3; ++c) ym[c] = rw(ydn + z, c)[l64];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (wave < 3) yu[c] = rows[par][wave + 1][c][l64];
+        if (wave > 0) ym[c] = rows[par][wave - 1][c][l64];
+      }
+      const int su = __builtin_amdgcn_readfirstlane(s);
+      const int dn[3] = {__builtin_amdgcn_readfirstlane(xdn + z), __builtin_amdgcn_readfirstlane(ydn + z),
+                         __builtin_amdgcn_readfirstlane(zdn)};
+      double2 acc[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) acc[c] = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int mu_ = 0; mu_ < 3; ++mu_) {
+        const double2* nu = mu_ == 0 ? xu : (mu_ == 1 ? yu : zn);
+        const double2* nd = mu_ == 0 ? xm : (mu_ == 1 ? ym : zm);
+        const double2* Uf = Ut + ((size_t)su * 3 + mu_) * 9;
+        const double2* Ub = Ut + ((size_t)dn[mu_] * 3 + mu_) * 9;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+#pragma unroll
+          for (int bb = 0; bb < 3; ++bb) {
+            acc[a] = cfma(Uf[3 * a + bb], nu[bb], acc[a]);
+            acc[a] = cfmac(Ub[3 * bb + a], nd[bb], acc[a]);
+          }
+        }
+      }
+      if (!RES) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          double2 o;
+          o.x = acc[c].x - 6.0 * zc[c].x;
+          o.y = acc[c].y - 6.0 * zc[c].y;
+          st_nt(out + (size_t)ts * V3 * 3 * K + cb * 64 + (unsigned)((3 * su + c) * K) + l64, o);
+        }
+      } else {
+        // block-major filter storage: block cb*64/BW, offset (cb*64) % BW + l64 within its rows
+        const size_t n = (size_t)3 * V3;
+        const size_t rbase = ((size_t)ts * (K / BW) + (cb * 64) / BW) * n * BW + (cb * 64) % BW;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const float2 rv =
+              make_float2((float)(cs * (acc[a].x + mc * zc[a].x)), (float)(cs * (acc[a].y + mc * zc[a].y)));
+          const size_t ri = rbase + (unsigned)((3 * su + a) * BW) + l64;
+          if (BF16)
+            static_cast<unsigned*>(rbuf)[ri] = bf2_pack(rv);
+          else
+            static_cast<float2*>(rbuf)[ri] = rv;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        zm[c] = zc[c];
+        zc[c] = zn[c];
+      }
     }
   }
 }
@@ -931,7 +933,7 @@ hipError_t launch_filter_step(const float2* U32, const void* in_, const void* r_
     float4* out4 = reinterpret_cast<float4*>(out);
     if (zrun > 0) {
       if (L % zrun != 0 || W % 4 != 0) return hipErrorInvalidValue;
-      const dim3 gy((unsigned)(V3 / (4 * zrun)) * (unsigned)(K / 128) * (unsigned)nts);
+      const dim3 gy((unsigned)(V3 / (4 * L)) * (unsigned)(K / 128) * (unsigned)nts);
 #define LAPH_YGROUP_CASE(NZ)                                                                                   \
   case NZ:                                                                                                     \
     if (r && old)                                                                                              \
@@ -977,7 +979,7 @@ hipError_t launch_filter_step(const float2* U32, const void* in_, const void* r_
 template <bool RES, int BW, bool BF16>
 static hipError_t launch_lap64y(const double2* U64, const double2* v, double2* out, const double* mu, double lamL,
                                 void* r, int L, int K, int nts, const int* first_blk, int W, int zrun, hipStream_t st) {
-  const dim3 g((unsigned)(L * L * L / (4 * zrun)) * (unsigned)(K / 64) * (unsigned)nts);
+  const dim3 g((unsigned)(L * L / 4) * (unsigned)(K / 64) * (unsigned)nts);
   switch (zrun) {
     case 1:
       k_lap64y<RES, BW, BF16, 1><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W);
