@@ -675,8 +675,9 @@ __global__ __launch_bounds__(256) void k_lap_residual64(const double2* __restric
 
 // The FP64 stencils on 64-column blocks in work-groups of 4 neighbouring
 // y-rows of one x-plane of a sweep strip (W a multiple of 4), each wavefront
-// walking the z-line of its row in unrolled runs of NZ sites, as the filter
-// step k_filter_step32x2y: z-neighbour rows shifted through registers,
+// walking zspan consecutive z-sites of its row (a multiple of NZ dividing L)
+// in unrolled runs of NZ sites, as the filter step k_filter_step32x2y:
+// z-neighbour rows shifted through registers,
 // y-neighbours of the inner rows read from the neighbouring wavefronts' LDS
 // copy of their centre rows (one barrier per site, LDS double-buffered by
 // site parity), row addresses formed in scalar registers.  RES = false:
@@ -687,17 +688,19 @@ template <bool RES, int BW, bool BF16, int NZ>
 __global__ __launch_bounds__(256) void k_lap64y(const double2* __restrict__ U64, const double2* __restrict__ v,
                                                 double2* __restrict__ out, const double* __restrict__ mu,
                                                 double lamL, void* __restrict__ rbuf, int L, int K, int nts,
-                                                const int* __restrict__ first_blk, int W) {
+                                                const int* __restrict__ first_blk, int W, int zspan) {
   __shared__ double2 rows[2][4][3][64];
   const int V3 = L * L * L;
-  const int nyb = W / 4;
+  const int nyb = W / 4, nzr = L / zspan;
   int ts, cb, g;
-  unit_map(V3 / (4 * L), K / 64, nts, ts, cb, g);
+  unit_map(V3 / (4 * zspan), K / 64, nts, ts, cb, g);
   if (first_blk && cb < first_blk[ts]) return;  // whole work-group
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int yb = g % nyb;
-  const int t = g / nyb;
+  const int zr = g % nzr;
+  const int yb = (g / nzr) % nyb;
+  const int t = g / nzr / nyb;
   const int x = t % L, strip = t / L;
+  const int zbeg = zr * zspan;
   const int y = strip * W + yb * 4 + wave;
   const int l64 = threadIdx.x & 63;
   const double2* vb = v + (size_t)ts * V3 * 3 * K + cb * 64;  // column block of the timeslice
@@ -719,15 +722,15 @@ __global__ __launch_bounds__(256) void k_lap64y(const double2* __restrict__ U64,
   {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      zm[c] = rw(line + L - 1, c)[l64];
-      zc[c] = rw(line, c)[l64];
+      zm[c] = rw(line + (zbeg == 0 ? L - 1 : zbeg - 1), c)[l64];
+      zc[c] = rw(line + zbeg, c)[l64];
     }
   }
-  for (int z0 = 0; z0 < L; z0 += NZ) {
+  for (int z0 = zbeg; z0 < zbeg + zspan; z0 += NZ) {
 #pragma unroll
     for (int k = 0; k < NZ; ++k) {
       const int z = z0 + k;
-      const int par = z & 1;
+      const int par = (z - zbeg) & 1;
       const int s = line + z;
       const int zdn = line + (z == 0 ? L - 1 : z - 1);
       const int zup = line + (z == L - 1 ? 0 : z + 1);
@@ -976,22 +979,25 @@ hipError_t launch_filter_step(const float2* U32, const void* in_, const void* r_
   return hipGetLastError();
 }
 
+// D X walks runs of zrun sites per work-group (measured faster for it than
+// whole lines), the residual kernel whole z-lines.
 template <bool RES, int BW, bool BF16>
 static hipError_t launch_lap64y(const double2* U64, const double2* v, double2* out, const double* mu, double lamL,
                                 void* r, int L, int K, int nts, const int* first_blk, int W, int zrun, hipStream_t st) {
-  const dim3 g((unsigned)(L * L / 4) * (unsigned)(K / 64) * (unsigned)nts);
+  const int zspan = RES ? L : zrun;
+  const dim3 g((unsigned)(L * L * L / (4 * zspan)) * (unsigned)(K / 64) * (unsigned)nts);
   switch (zrun) {
     case 1:
-      k_lap64y<RES, BW, BF16, 1><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W);
+      k_lap64y<RES, BW, BF16, 1><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W, zspan);
       break;
     case 2:
-      k_lap64y<RES, BW, BF16, 2><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W);
+      k_lap64y<RES, BW, BF16, 2><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W, zspan);
       break;
     case 4:
-      k_lap64y<RES, BW, BF16, 4><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W);
+      k_lap64y<RES, BW, BF16, 4><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W, zspan);
       break;
     case 8:
-      k_lap64y<RES, BW, BF16, 8><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W);
+      k_lap64y<RES, BW, BF16, 8><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W, zspan);
       break;
     default:
       return hipErrorInvalidValue;
