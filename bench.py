@@ -40,12 +40,12 @@ LAMBDA_L = 12.0
 PEAK_HBM = 8.0e12
 PEAK_FP64_MFMA = 78.6e12  # dense FP64 matrix peak of the MI355X
 # HBM bytes per filter-step launch (all columns active, the 64 timeslices of
-# the default single lane, 128-vector blocks in y-row work-groups with z-runs
-# of 8, y-strips of 4) from rocprofv3
+# the default single lane, 128-vector blocks in y-row work-groups walking
+# whole z-lines, y-strips of 4) from rocprofv3
 # --pmc FETCH_SIZE / WRITE_SIZE on the isolated replays of --profile-filter,
 # each scaled by the same counter on a 1 GiB device copy (python bench.py
 # --summarize-pmc ...); None = not measured for this build.
-TRAFFIC_BYTES_PER_LAUNCH = 45839473743  # profiles/filter_step_32x32x32x64_k256_pmc.md
+TRAFFIC_BYTES_PER_LAUNCH = 42789374300  # profiles/filter_step_32x32x32x64_k256_pmc.md
 
 
 def main():
@@ -155,7 +155,7 @@ def main():
     achieved = bytes_launch / (ms_launch * 1e-3)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                 "frac": achieved / PEAK_HBM, "traffic": TRAFFIC_BYTES_PER_LAUNCH, "traffic_unit": "bytes/launch",
-                "kernel": (f"k_filter_step32x2y (block 128, y-row groups, z-runs of {st.filter_zrun})"
+                "kernel": (f"k_filter_step32x2y (block 128, y-row groups walking z-lines in runs of {st.filter_zrun})"
                            if st.filter_zrun else
                            f"k_filter_step32{'x2' if st.filter_block == 128 else ''} (block {st.filter_block})"),
                 "avg_launch_us": ms_launch * 1e3, "bytes_per_launch": bytes_launch,
