@@ -212,12 +212,13 @@ def test_shifted_cholqr_path(lp, gpu, monkeypatch):
 
 
 @pytest.mark.parametrize("L,nts,nev,nsub,lanes", [(5, 3, 16, 64, 2), (7, 1, 30, 64, 1), (9, 2, 60, 64, 2),
-                                                   (6, 4, 20, 64, 3)])
+                                                   (6, 4, 20, 64, 3), (12, 1, 40, 128, 1)])
 def test_batch_edge_shapes(lp, gpu, L, nts, nev, nsub, lanes, monkeypatch):
-    """Odd L and L not divisible by 4 (plain sweep order, partial last
-    site-block), a single timeslice, timeslices split over two or three
-    lanes (host threads and streams, LAPH_LANES), and nev close to the
-    search-space width."""
+    """Odd L and L not divisible by 4 (plain sweep order, one site per
+    wavefront, partial last site-block), L = 12 (y-row work-groups walking
+    z-lines in runs of 4, three strips of 4 or x-major planes), a single
+    timeslice, timeslices split over two or three lanes (host threads and
+    streams, LAPH_LANES), and nev close to the search-space width."""
     tol = 1e-10
     data = smeared(L, 4000 + L, 0.4, nts)
     monkeypatch.setenv("LAPH_LANES", str(lanes))
@@ -225,7 +226,8 @@ def test_batch_edge_shapes(lp, gpu, L, nts, nev, nsub, lanes, monkeypatch):
     for t, (ts, _) in enumerate(data):
         b.set_timeslice(t, ts, 0.62, 0.62, 3)
     st = b.solve()
-    assert st.converged == nts and st.filter_strip in ((L,) if L % 4 else (L, 4))
+    assert st.converged == nts and st.filter_strip in ((L,) if L % 4 else (L, 4, 8))
+    assert st.filter_zrun == (0 if L % 4 or st.filter_block != 128 else (8 if L % 8 == 0 else 4))
     for t, (_, U) in enumerate(data):
         ev, V, ph, res = b.get(t)
         check_pairs(ev, V, res, N.dense_D(U.reshape(L, L, L, 3, 3, 3)), nev, tol)
