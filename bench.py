@@ -191,7 +191,7 @@ def main():
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32 filter + fp64 CholQR2/RR/lock", "data": "synthetic",
         "config": {"workload": f"{L}^3x{nts} eps={EPS} HYP 0.62/0.62/3, N_ev={NEV}, Chebyshev degree {DEG}, "
-                               f"one filter pass per cycle of degree <= 72 after two of degree {DEG}, vector block {st.filter_block}, "
+                               f"one filter pass per cycle of degree <= 72 after one of degree {DEG}, vector block {st.filter_block}, "
                                f"sweep strip {st.filter_strip}, search space {NSUB}, "
                                f"residual <= {TOL}",
                    "lattice": f"{L}^3x{nts}", "nev": NEV, "deg": DEG, "tol": TOL, "parallelism": "none (1 GPU)"},

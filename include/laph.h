@@ -180,7 +180,9 @@ typedef struct laph_batch laph_batch;
  *                           pass as a product over the roots of (T_m(y) - T_m(theta)) / (y - theta)
  *                           applied to r (2); the same with the correction pass reading r in every
  *                           step (1); the three-term recurrence, which also reads the previous vector (0)
- *   LAPH_WARM=n             single-pass cycles before T_m(B)^P cycles (2)
+ *   LAPH_WARM=n             cycles of T_DEG(B) before the per-cycle degree (or T_m(B)^P) cycles (1)
+ *   LAPH_WINDOW0=x          window lower end of the first cycle from the start block (3 x the K-th
+ *                           level of the free Laplacian, at most 0.75 lambda_l)
  *   LAPH_QR_CYCLES=n        cycles (and start block) orthonormalised by explicit CholQR2 (0)
  *   LAPH_START_RR=1         Rayleigh-Ritz on the Gaussian start block before the first cycle (off: the
  *                           first cycle filters the block as it is)
@@ -203,7 +205,7 @@ typedef struct laph_batch_opts {
   int max_iter;      /* filter / Rayleigh-Ritz cycles (0 = 60) */
   unsigned long long seed;
   int warm_start;    /* 1: start from the block proposed by laph_net_propose, when one was */
-  int filter_passes; /* 0: after two warm-up cycles of T_DEG(B), one Chebyshev filter per cycle of
+  int filter_passes; /* 0: after one warm-up cycle of T_DEG(B), one Chebyshev filter per cycle of
                         the highest degree (a multiple of 4, at most 72) whose amplification
                         T_d(theta_max) stays below 1e10; 1 .. 6: up to that many applications of
                         T_DEG(B) per cycle, fewer while T_DEG(theta_max)^P would exceed 1e10 */

@@ -464,16 +464,34 @@ static int orthonormalise_and_rr(const View& v, Act q, bool explicit_qr) {
   return set_error(LAPH_ERR_NOCONV, "Rayleigh-Ritz: Gram matrix not positive definite after CholQR2");
 }
 
+// K-th lowest eigenvalue of -D on the free field (every link 1): the levels
+// 4 sum_i sin^2(pi k_i / L) of the L^3 momenta, each three-fold (colour).
+// HYP-smeared fields lie close above it (at 32^3 x 256 vectors, eps 0.3: 0.304
+// free, 0.31 for the smeared field).
+double free_level(int L, int K) {
+  std::vector<double> s(L), ev;
+  ev.reserve((size_t)L * L * L);
+  for (int k = 0; k < L; ++k) s[k] = 4.0 * std::pow(std::sin(M_PI * k / L), 2);
+  for (int a = 0; a < L; ++a)
+    for (int c = 0; c < L; ++c)
+      for (int d = 0; d < L; ++d) ev.push_back(s[a] + s[c] + s[d]);
+  const size_t idx = std::min((size_t)(K - 1) / 3, ev.size() - 1);
+  std::nth_element(ev.begin(), ev.begin() + idx, ev.end());
+  return ev[idx];
+}
+
 // Start of a solve from the Gaussian block without a Rayleigh-Ritz step: a
 // rotation of the block does not change its span and the filter is linear,
 // so the first (single-pass) filter cycle can take the columns as they are,
 // each with theta = 1, the window edge (T_m(1) = 1, gamma = 1: the filtered
-// block is X + e).  Needs D X for the residuals; the window lower end sits at
-// its cap, 0.75 lambda_l, which a random block's largest Ritz value exceeds.
+// block is X + e).  Needs D X for the residuals.  The window lower end of
+// this first cycle is laph_batch::window0, three times the K-th level of the
+// free Laplacian (the largest Ritz value of a random block would sit near the
+// top of the spectrum and make the first filter a weak low-pass).
 static int start_from_block(const View& v) {
   laph_batch* b = v.b;
   const size_t nk = (size_t)v.ln->nt * b->K;
-  std::fill(b->h_mu.begin() + v.nk(), b->h_mu.begin() + v.nk() + nk, 0.75 * b->p.lambda_l);
+  std::fill(b->h_mu.begin() + v.nk(), b->h_mu.begin() + v.nk() + nk, b->window0);
   LAPH_HIP(hipMemcpyAsync(b->mu + v.nk(), b->h_mu.data() + v.nk(), nk * sizeof(double), hipMemcpyHostToDevice,
                           v.ln->st));
   LAPH_HIP(launch_lap_residual64(b->U64 + v.lo(), v.X(), b->mu + v.nk(), b->p.lambda_l, fb(b, b->R32, v.vo()),
@@ -1235,6 +1253,13 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     if (const char* e = std::getenv("LAPH_TRSM")) b->use_trsm = std::atoi(e) != 0;
     if (const char* e = std::getenv("LAPH_AMP_CAP")) b->amp_cap = std::max(10.0, std::atof(e));
     b->zrun = zrun;
+    // first-cycle window: 3 x the free Laplacian's K-th level (measured best
+    // between 2.5 and 3.5 x at 24^3, 32^3 and 48^3), at most the usual cap
+    b->window0 = std::min(3.0 * laph::free_level(p->L, K), 0.75 * p->lambda_l);
+    if (const char* e = std::getenv("LAPH_WINDOW0")) {
+      const double w = std::atof(e);
+      if (w > 0.0) b->window0 = std::min(w, 0.75 * p->lambda_l);
+    }
     b->roots = laph::leja_roots(p->deg);
     b->strip = strip > 0 ? strip : p->L;
     b->strip_fixed = strip > 0;

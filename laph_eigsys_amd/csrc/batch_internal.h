@@ -90,7 +90,8 @@ struct laph_batch {
   // the candidates (LAPH_BW fixes it)
   int bw = 0;
   int bf16 = 0;  // filter vectors stored as BF16 (laph_batch_opts.filter_storage)
-  int warm_cycles = 2;  // single-pass cycles while the window settles (LAPH_WARM)
+  int warm_cycles = 1;  // cycles of the batch's degree while the window settles (LAPH_WARM)
+  double window0 = 0;   // window lower end of the first cycle from the start block (LAPH_WINDOW0)
   int strip = 0;  // sweep strip width of the stencil kernels (divides L; L = x-major planes)
   bool strip_fixed = false;  // LAPH_STRIP given
   int zrun = 0;  // z-sites per wavefront of the y-row work-group filter step (LAPH_ZRUN; 0: one site each)
@@ -122,6 +123,8 @@ struct laph_batch {
 };
 
 namespace laph {
+// K-th lowest eigenvalue of -D on the free field (all links 1).
+double free_level(int L, int K);
 // z-sites per wavefront of the y-row work-group stencils for the batch's sweep
 // strip (they need a strip width that is a multiple of 4); 0: one site each.
 inline int stencil_zrun(const laph_batch* b) { return b->strip % 4 == 0 ? b->zrun : 0; }
