@@ -684,13 +684,14 @@ __global__ __launch_bounds__(256) void k_lap_residual64(const double2* __restric
 // out = D v in the site-major layout; RES = true: the Ritz residuals
 // r = c_t (D v + v diag(mu)) in the filter storage (BW-vector blocks, FP32
 // or BF16), as k_lap_residual64.
-template <bool RES, int BW, bool BF16, int NZ>
+template <bool RES, int BW, bool BF16, int NZ, bool LINE>
 __global__ __launch_bounds__(256) void k_lap64y(const double2* __restrict__ U64, const double2* __restrict__ v,
                                                 double2* __restrict__ out, const double* __restrict__ mu,
                                                 double lamL, void* __restrict__ rbuf, int L, int K, int nts,
-                                                const int* __restrict__ first_blk, int W, int zspan) {
+                                                const int* __restrict__ first_blk, int W) {
   __shared__ double2 rows[2][4][3][64];
   const int V3 = L * L * L;
+  const int zspan = LINE ? L : NZ;  // z-sites per work-group
   const int nyb = W / 4, nzr = L / zspan;
   int ts, cb, g;
   unit_map(V3 / (4 * zspan), K / 64, nts, ts, cb, g);
@@ -984,7 +985,7 @@ hipError_t launch_filter_step(const float2* U32, const void* in_, const void* r_
   return hipGetLastError();
 }
 
-// D X walks runs of zrun sites per work-group (measured faster for it than
+// D X walks one run of zrun sites per work-group (measured faster for it than
 // whole lines), the residual kernel whole z-lines.
 template <bool RES, int BW, bool BF16>
 static hipError_t launch_lap64y(const double2* U64, const double2* v, double2* out, const double* mu, double lamL,
@@ -993,16 +994,16 @@ static hipError_t launch_lap64y(const double2* U64, const double2* v, double2* o
   const dim3 g((unsigned)(L * L * L / (4 * zspan)) * (unsigned)(K / 64) * (unsigned)nts);
   switch (zrun) {
     case 1:
-      k_lap64y<RES, BW, BF16, 1><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W, zspan);
+      k_lap64y<RES, BW, BF16, 1, RES><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W);
       break;
     case 2:
-      k_lap64y<RES, BW, BF16, 2><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W, zspan);
+      k_lap64y<RES, BW, BF16, 2, RES><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W);
       break;
     case 4:
-      k_lap64y<RES, BW, BF16, 4><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W, zspan);
+      k_lap64y<RES, BW, BF16, 4, RES><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W);
       break;
     case 8:
-      k_lap64y<RES, BW, BF16, 8><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W, zspan);
+      k_lap64y<RES, BW, BF16, 8, RES><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W);
       break;
     default:
       return hipErrorInvalidValue;
