@@ -181,6 +181,8 @@ typedef struct laph_batch laph_batch;
  *                           applied to r (2); the same with the correction pass reading r in every
  *                           step (1); the three-term recurrence, which also reads the previous vector (0)
  *   LAPH_WARM=n             cycles of T_DEG(B) before the per-cycle degree (or T_m(B)^P) cycles (1)
+ *   LAPH_WARM_DEG=d         degree of the warm-up cycles in the per-cycle degree mode (DEG; a multiple
+ *                           of 4 up to LAPH_FILTER_DEGREE)
  *   LAPH_WINDOW0=x          window lower end of the first cycle from the start block (3 x the K-th
  *                           level of the free Laplacian, at most 0.75 lambda_l)
  *   LAPH_QR_CYCLES=n        cycles (and start block) orthonormalised by explicit CholQR2 (0)

@@ -983,7 +983,8 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
     // one filter pass while the window settles, then T_m(B)^P per cycle; in
     // the per-cycle degree mode one pass of the degree the window allows
     const bool warmup = it < b->warm_cycles;
-    ln->m = b->fdeg_max > 0 && !warmup ? cycle_degree(b, lc) : b->deg;
+    const bool wdeg = b->fdeg_max >= b->warm_deg && b->warm_deg > 0;  // warm-up degree of the per-cycle mode
+    ln->m = b->fdeg_max > 0 && !warmup ? cycle_degree(b, lc) : (warmup && wdeg ? b->warm_deg : b->deg);
     const int passes = b->fdeg_max > 0 || warmup ? 1 : filter_passes(b, lc);
     const int m = ln->m;
     LAPH_TRY(upload_coefficients(v, lc, passes));
@@ -1234,6 +1235,7 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     b->per_ts = (size_t)n * K;
     b->trace = std::getenv("LAPH_TRACE") != nullptr;
     if (const char* e = std::getenv("LAPH_WARM")) b->warm_cycles = std::max(0, std::atoi(e));
+    if (const char* e = std::getenv("LAPH_WARM_DEG")) b->warm_deg = std::max(0, std::atoi(e)) / 4 * 4;
     b->force_shift = std::getenv("LAPH_SHIFTED_CHOLQR") != nullptr;
     if (const char* e = std::getenv("LAPH_QR_CYCLES")) b->qr_cycles = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("LAPH_TEST_POISON")) b->poison_ts = std::atoi(e);

@@ -92,6 +92,7 @@ struct laph_batch {
   int bf16 = 0;  // filter vectors stored as BF16 (laph_batch_opts.filter_storage)
   int warm_cycles = 1;  // cycles of the batch's degree while the window settles (LAPH_WARM)
   double window0 = 0;   // window lower end of the first cycle from the start block (LAPH_WINDOW0)
+  int warm_deg = 0;     // degree of the warm-up cycles in the per-cycle degree mode (0: the batch's degree)
   int strip = 0;  // sweep strip width of the stencil kernels (divides L; L = x-major planes)
   bool strip_fixed = false;  // LAPH_STRIP given
   int zrun = 0;  // z-sites per wavefront of the y-row work-group filter step (LAPH_ZRUN; 0: one site each)
