@@ -330,8 +330,10 @@ __device__ __forceinline__ const float4* row(const float4* p, int site, int c) {
 // the neighbouring wavefronts and only the outer rows of the group load them:
 // 10.5 instead of 21 global row loads per site.  The work-groups of one unit
 // run y-block, then x, then strip, so the lines in flight on an XCD are the
-// neighbouring x-planes of one strip, all advancing along z.  One work-group barrier per
-// site (LDS double-buffered by site parity).  Lanes of a locked 64-vector half
+// neighbouring x-planes of one strip, all advancing along z.  One work-group
+// barrier per site (LDS double-buffered by site parity: a buffer is rewritten
+// two sites later, after every wavefront has passed the barrier between).
+// Lanes of a locked 64-vector half
 // (only in the one block per timeslice that straddles the lock boundary) load
 // and compute like the others and only skip their stores, so the wavefronts
 // never diverge.  Site indices are wave-uniform: row addresses and links are
