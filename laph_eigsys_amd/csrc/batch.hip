@@ -446,7 +446,8 @@ static int orthonormalise_and_rr(const View& v, Act q, bool explicit_qr) {
         // even the shifted pass failed (a block that is not finite): the
         // failing timeslices (non-zero info) restart their active columns
         // from Gaussian vectors; the others keep theirs
-        if (b->trace) std::fprintf(stderr, "[laph] lane %d: %s; active columns re-seeded\n", v.ln->t0, laph_last_error());
+        if (b->trace)
+          std::fprintf(stderr, "[laph] lane %d: %s; active columns re-seeded\n", v.ln->t0, laph_last_error());
         LAPH_HIP(launch_reseed_cols(v.X(), b->n, b->K, v.ln->nt, q.l, b->info + v.ln->t0,
                                     b->o.seed + 0x5851F42Dull * (++v.ln->recoveries), v.ln->st));
         rc = cholqr(v, q);
@@ -760,7 +761,8 @@ static int enqueue_filter(const View& v) {
     void* out = fb(b, b->P[(st - 1) % nb], vo);
     LAPH_HIP(launch_filter_step(b->U32 + v.lo(), in, b->qform ? nullptr : R, old, out,
                                 b->coef + (st - 1) * step + v.nk(),
-                                b->cab + v.ln->t0, b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, b->strip, stencil_zrun(b), v.ln->st));
+                                b->cab + v.ln->t0, b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, b->strip,
+                                stencil_zrun(b), v.ln->st));
   }
   return LAPH_OK;
 }
@@ -835,7 +837,8 @@ static int enqueue_plain_pass(const View& v, int p) {
     const bool last = st == m;
     const float4* co = b->coef2 + (size_t)(last ? m - 1 + p - 2 : st - 1) * step + v.nk();
     LAPH_HIP(launch_filter_step(b->U32 + v.lo(), in, last ? f : nullptr, old, out, co, b->cab + v.ln->t0,
-                                b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, b->strip, stencil_zrun(b), v.ln->st));
+                                b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, b->strip, stencil_zrun(b),
+                                v.ln->st));
   }
   return LAPH_OK;
 }
@@ -1040,7 +1043,8 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
       const double worst = worst_residual(b, ln);
       const double* mu = &b->h_mu[0];
       const double* rs = &b->h_res[0];
-      std::fprintf(stderr, "[laph] cycle %d: degree %d passes %d  window %.4f  mu[nev-1] %.5f  mu[K-1] %.4f  res[0] %.2e  "
+      std::fprintf(stderr,
+                   "[laph] cycle %d: degree %d passes %d  window %.4f  mu[nev-1] %.5f  mu[K-1] %.4f  res[0] %.2e  "
                    "res[nev-1] %.2e  worst %.2e  locked %d  lock %d\n", it, m, passes, lc[0], mu[b->nev - 1], mu[K - 1],
                    std::sqrt(rs[0]), std::sqrt(rs[b->nev - 1]), worst, b->nlock[0], ln->lock);
     }
