@@ -95,6 +95,33 @@ def test_batch_matches_plain_fp64(lp, gpu):
             assert np.abs(V[i] - Vpf[i]).max() < 1e-9, i
 
 
+@pytest.mark.parametrize("window0", [None, "0.3"])
+def test_rough_field_start_window(lp, gpu, window0, monkeypatch):
+    """An unsmeared, strongly disordered field (eps 1.2, no HYP; lambda_K =
+    2.08 for K = 64 at L = 8, the free Laplacian's 1.76) from the default
+    first-cycle window (three times the free K-th level) and from a window
+    far below the field's lambda_K (LAPH_WINDOW0 = 0.3), where the first
+    filter damps wanted directions: the next windows come from the Ritz
+    values and both give the dense oracle's spectrum."""
+    L, nts, nev, deg, tol = 8, 2, 24, 8, 1e-10
+    if window0:
+        monkeypatch.setenv("LAPH_WINDOW0", window0)
+    cfg = N.synth_config(77, 1.2, L, nts)
+    b = lp.Batch(L, deg, 12.0, 0.0, nts, nev, nsub=64, tol=tol, seed=9, device=gpu)
+    data = []
+    for t in range(nts):
+        ts = cfg[t * 72 * L ** 3:(t + 1) * 72 * L ** 3]
+        b.set_timeslice(t, ts, 0.62, 0.62, 0)
+        data.append(O.map_timeslice(ts, L))
+    st = b.solve()
+    print(f"window0 {window0}: cycles {st.iterations} max res {st.max_residual:.2e}")
+    assert st.converged == nts
+    for t, U in enumerate(data):
+        ev, V, ph, res = b.get(t)
+        check_pairs(ev, V, res, N.dense_D(U.reshape(L, L, L, 3, 3, 3)), nev, tol)
+    b.close()
+
+
 @pytest.mark.parametrize("storage,width", [(0, 64), (0, 128), (1, 128), (1, 256)])
 def test_filter_block_widths(lp, gpu, storage, width, monkeypatch):
     """Every layout of the filter buffers (FP32 storage: one or two vectors
