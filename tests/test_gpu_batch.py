@@ -95,6 +95,30 @@ def test_batch_matches_plain_fp64(lp, gpu):
             assert np.abs(V[i] - Vpf[i]).max() < 1e-9, i
 
 
+@pytest.mark.parametrize("strip", [8, 16])
+def test_batch_matches_plain_fp64_l16(lp, gpu, strip, monkeypatch):
+    """Mixed batched vs plain FP64 Krylov-Schur on two 16^3 timeslices with
+    the y-row work-group stencils over y-strips of 8 (two groups per strip
+    and x-plane) and over whole x-planes."""
+    L, nts, nev, deg = 16, 2, 60, 8
+    monkeypatch.setenv("LAPH_STRIP", str(strip))
+    data = smeared(L, 5151, 0.35, nts)
+    b = lp.Batch(L, deg, 12.0, 0.0, nts, nev, nsub=128, tol=1e-10, seed=2, device=gpu)
+    for t, (ts, _) in enumerate(data):
+        b.set_timeslice(t, ts, 0.62, 0.62, 3)
+    st = b.solve()
+    assert st.converged == nts and st.filter_strip == strip
+    for t, (ts, _) in enumerate(data):
+        ev, V, ph, res = b.get(t)
+        lamC = -ev[-1] * 1.3
+        with lp.Tslice(L, deg, 12.0, lamC, device=gpu, ts_config=ts) as tsl:
+            tsl.smearing_hyp(0.62, 0.62, 3)
+            theta, Vp, _ = tsl.eps_solve(nev, tol=1e-12)
+        evp = lp.recover_spectrum(theta, deg, 12.0, lamC)
+        np.testing.assert_allclose(ev, evp, rtol=1e-10, atol=1e-12)
+    b.close()
+
+
 @pytest.mark.parametrize("window0", [None, "0.3"])
 def test_rough_field_start_window(lp, gpu, window0, monkeypatch):
     """An unsmeared, strongly disordered field (eps 1.2, no HYP; lambda_K =
