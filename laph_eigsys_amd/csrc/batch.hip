@@ -1180,6 +1180,7 @@ static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
   s.filter_block = b->bw;
   s.filter_strip = b->strip;
   s.filter_zrun = b->bf16 == 0 && b->bw == 128 ? laph::stencil_zrun(b) : 0;
+  s.start_window = b->window0;
   s.converged = conv;
   s.max_residual = maxres;
   s.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -53,6 +53,11 @@ def test_batch_against_dense(lp, gpu, L, nts, nev, deg):
     st = b.solve()
     print(f"L={L} nts={nts}: cycles {st.iterations} fp32 vecs {st.stencil_vecs} max res {st.max_residual:.2e}")
     assert st.converged == nts
+    # first-cycle window: 3 x the K-th (K = 64) level of the free Laplacian,
+    # whose levels are 4 sum_i sin^2(pi k_i / L), each three-fold; at most 9
+    s2 = 4 * np.sin(np.pi * np.arange(L) / L) ** 2
+    free = np.sort(np.repeat((s2[:, None, None] + s2[None, :, None] + s2[None, None, :]).ravel(), 3))
+    assert abs(st.start_window - min(3 * free[63], 9.0)) < 1e-12
     for t, (_, U) in enumerate(data):
         ev, V, ph, res = b.get(t)
         D = N.dense_D(U.reshape(L, L, L, 3, 3, 3))
