@@ -270,13 +270,14 @@ def test_shifted_cholqr_path(lp, gpu, monkeypatch):
 
 
 @pytest.mark.parametrize("L,nts,nev,nsub,lanes", [(5, 3, 16, 64, 2), (7, 1, 30, 64, 1), (9, 2, 60, 64, 2),
-                                                   (6, 4, 20, 64, 3), (12, 1, 40, 128, 1)])
+                                                   (6, 4, 20, 64, 3), (12, 1, 40, 128, 1), (8, 3, 40, 128, 2)])
 def test_batch_edge_shapes(lp, gpu, L, nts, nev, nsub, lanes, monkeypatch):
     """Odd L and L not divisible by 4 (plain sweep order, one site per
     wavefront, partial last site-block), L = 12 (y-row work-groups walking
     z-lines in runs of 4, three strips of 4 or x-major planes), a single
     timeslice, timeslices split over two or three lanes (host threads and
-    streams, LAPH_LANES), and nev close to the search-space width."""
+    streams, LAPH_LANES; at L = 8 with the y-row work-group kernels on each
+    lane's share), and nev close to the search-space width."""
     tol = 1e-10
     data = smeared(L, 4000 + L, 0.4, nts)
     monkeypatch.setenv("LAPH_LANES", str(lanes))
