@@ -407,12 +407,8 @@ __global__ __launch_bounds__(256) void k_filter_step32x2y(const float2* __restri
 #pragma unroll
         for (int c = 0; c < 3; ++c) ym[c] = row(v, ydn + z, c)[l64];
       }
-      __syncthreads();
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        if (wave < 3) yu[c] = rows[par][wave + 1][c][l64];
-        if (wave > 0) ym[c] = rows[par][wave - 1][c][l64];
-      }
+      // x and z terms need no other wavefront's rows: computed before the
+      // barrier, so a wavefront waits there with less work left after it
       float2 acc0[3], acc1[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -422,8 +418,14 @@ __global__ __launch_bounds__(256) void k_filter_step32x2y(const float2* __restri
       const int su = __builtin_amdgcn_readfirstlane(s), xd = __builtin_amdgcn_readfirstlane(xdn + z);
       const int yd = __builtin_amdgcn_readfirstlane(ydn + z), zd = __builtin_amdgcn_readfirstlane(zdn);
       link_terms2(Ut + ((size_t)su * 3 + 0) * 9, Ut + ((size_t)xd * 3 + 0) * 9, xu, xm, acc0, acc1);
-      link_terms2(Ut + ((size_t)su * 3 + 1) * 9, Ut + ((size_t)yd * 3 + 1) * 9, yu, ym, acc0, acc1);
       link_terms2(Ut + ((size_t)su * 3 + 2) * 9, Ut + ((size_t)zd * 3 + 2) * 9, zn, zm, acc0, acc1);
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (wave < 3) yu[c] = rows[par][wave + 1][c][l64];
+        if (wave > 0) ym[c] = rows[par][wave - 1][c][l64];
+      }
+      link_terms2(Ut + ((size_t)su * 3 + 1) * 9, Ut + ((size_t)yd * 3 + 1) * 9, yu, ym, acc0, acc1);
       float4 o[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
