@@ -45,7 +45,7 @@ PEAK_FP64_MFMA = 78.6e12  # dense FP64 matrix peak of the MI355X
 # --pmc FETCH_SIZE / WRITE_SIZE on the isolated replays of --profile-filter,
 # each scaled by the same counter on a 1 GiB device copy (python bench.py
 # --summarize-pmc ...); None = not measured for this build.
-TRAFFIC_BYTES_PER_LAUNCH = 42841709238  # profiles/filter_step_32x32x32x64_k256_pmc.md
+TRAFFIC_BYTES_PER_LAUNCH = 43334465271  # profiles/filter_step_32x32x32x64_k256_pmc.md
 
 
 def main():
