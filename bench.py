@@ -4,10 +4,13 @@ mixed-precision batched solver vs the plain FP64 drop-in, on one MI355X.
 A step is one batched eigensolve of all 64 timeslices of the seeded
 synthetic configuration (U = exp(i 0.3 H), HYP 0.62/0.62/3 applied on the
 device beforehand; smeared links resident in HBM when the timed region
-starts): FP32 Chebyshev filter in residual-correction form (two cycles of
-degree 16, then one filter per cycle of the degree the window allows, up to
-72) on 64- or 128-vector blocks, FP64 Rayleigh-Ritz / residual locking, until
-every timeslice has 220 locked pairs with |D v - lambda v| <= 1e-10.
+starts): FP32 Chebyshev filter in residual-correction form (one warm-up
+cycle of degree 16, then one filter per cycle of the degree the window
+allows, up to 72) on 64- or 128-vector blocks, FP64 Rayleigh-Ritz / residual
+locking, until every timeslice has 220 locked pairs with
+|D v - lambda v| <= 1e-10, then fix_phase of every eigenvector: the timed
+region ends with phase-fixed eigenpairs resident in HBM.  A timed step that
+needed a recovery (a re-seeded block) fails the bench.
 
 Printed JSON (one line) also carries:
   roofline      the FP32 filter stencil (dominant kernel): algorithmic bytes
@@ -15,13 +18,16 @@ Printed JSON (one line) also carries:
                 captured filter alone on the device (HIP events on the
                 library's stream), vs 8 TB/s; `traffic` = measured HBM bytes
                 per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE on those
-                replays, each scaled by the same counter on a 1 GiB copy)
+                replays, each scaled by the same counter on a 1 GiB copy),
+                reported only while the kernel source is the one measured
   plain_fp64    the plain FP64 drop-in (device-resident Krylov-Schur on
-                T_16(B), lambda_c = 1.35 |lambda_220|) timed on a few
-                timeslices of the same configuration, timeslices/s
-  cpu_baseline  the oracle's Krylov-Schur (C, OpenMP) on one timeslice for
-                one restart cycle, scaled by the cycle count the same
-                algorithm needs on the GPU drop-in
+                T_16(B), lambda_c = 1.35 x the 220th free-field level, a rule
+                that needs no solve) on every timeslice of the same field,
+                timeslices/s, with the per-timeslice spread and the largest
+                relative eigenvalue difference to the batched solver
+  cpu_baseline  the oracle's Krylov-Schur (C, OpenMP over the host cores)
+                solving one timeslice to 1e-10, median of 3 solves;
+                --cpu-full runs the one-timeslice-per-core plan instead
 """
 import argparse
 import json
@@ -40,12 +46,13 @@ LAMBDA_L = 12.0
 PEAK_HBM = 8.0e12
 PEAK_FP64_MFMA = 78.6e12  # dense FP64 matrix peak of the MI355X
 # HBM bytes per filter-step launch (all columns active, the 64 timeslices of
-# the default single lane, 128-vector blocks in y-row work-groups walking
-# whole z-lines, y-strips of 4) from rocprofv3
-# --pmc FETCH_SIZE / WRITE_SIZE on the isolated replays of --profile-filter,
-# each scaled by the same counter on a 1 GiB device copy (python bench.py
-# --summarize-pmc ...); None = not measured for this build.
-TRAFFIC_BYTES_PER_LAUNCH = 43334465271  # profiles/filter_step_32x32x32x64_k256_pmc.md
+# the default single lane) from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the
+# isolated replays of --profile-filter, each scaled by the same counter on a
+# 1 GiB device copy (python bench.py --summarize-pmc ...), with the sha256
+# (first 16 hex digits) of the source file of the kernel it was measured on:
+# the figure is reported only while that file is unchanged.
+TRAFFIC = {"bytes_per_launch": 43334465271, "source": "laph_eigsys_amd/csrc/kernels_block.hip",
+           "sha16": "0000000000000000"}
 
 
 def main():
@@ -54,8 +61,14 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nts", type=int, default=LT)
-    ap.add_argument("--plain-ts", type=int, default=2)
+    ap.add_argument("--plain-ts", type=int, default=LT, help="timeslices solved by the plain FP64 drop-in")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="only the CPU baseline as planned: one timeslice per host core, each solved to 1e-10 by "
+                         "the oracle (single-threaded), --cpu-runs times")
+    ap.add_argument("--cpu-cores", type=int, default=0, help="with --cpu-full: cores (0 = OMP_NUM_THREADS or all)")
+    ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--cpu-worker", type=str, default=None, help=argparse.SUPPRESS)
     ap.add_argument("--filter-reps", type=int, default=20)
     ap.add_argument("--profile-filter", action="store_true",
                     help="counter-collection workload: filter replays + a calibration copy, no solve timing")
@@ -63,9 +76,6 @@ def main():
                     help="per-kernel durations from a rocprofv3 --kernel-trace database (no GPU use)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="HBM bytes per filter launch from two rocprofv3 --pmc runs of --profile-filter")
-    ap.add_argument("--gemm-solutions", action="store_true",
-                    help="only time every rocBLAS solution for the solver's large GEMM shapes "
-                         "(laph_eigsys_amd/gemm_solutions)")
     ap.add_argument("--summarize-mfma", metavar="PMC_CSV",
                     help="FP64 matrix-core rate per kernel from a rocprofv3 --pmc run with "
                          "SQ_INSTS_VALU_MFMA_MOPS_F64")
@@ -73,8 +83,8 @@ def main():
                     help="c3: the headline (32^3x64, N_ev 220); c2: 24^3x48 N_ev 120, every timeslice through the "
                          "plain drop-in and the batched solver; c4: 48^3x96 N_ev 200 in HBM-sized groups")
     ap.add_argument("--stream-ceiling", action="store_true",
-                    help="only run the streaming-bandwidth probe (laph_eigsys_amd/stream_ceiling)")
-    ap.add_argument("--no-stream", action="store_true", help="skip the streaming-bandwidth probe")
+                    help="only run the streaming-bandwidth measurement (laph_eigsys_amd/stream_ceiling)")
+    ap.add_argument("--no-stream", action="store_true", help="skip the streaming-bandwidth measurement")
     ap.add_argument("--last", type=int, default=0,
                     help="with --summarize-*: the isolated filter replays are the last LAST filter dispatches")
     args = ap.parse_args()
@@ -90,10 +100,12 @@ def main():
     if args.stream_ceiling:
         print(json.dumps(stream_ceiling()))
         return
-    if args.gemm_solutions:
-        import subprocess
-        exe = os.path.join(ROOT, "laph_eigsys_amd", "gemm_solutions")
-        sys.exit(subprocess.run([exe]).returncode)
+    if args.cpu_worker:
+        cpu_worker(args.cpu_worker)
+        return
+    if args.cpu_full:
+        print(json.dumps(cpu_full(args.cpu_cores, args.cpu_runs, args.nts)))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); timeslices are "
               "independent, so N GPUs would each run their own Lt/N share with no exchange.", file=sys.stderr)
@@ -129,7 +141,8 @@ def main():
         st = b.solve()
         print(f"[bench] warmup {w}: {st.seconds:.2f}s cycles={st.iterations}", file=sys.stderr, flush=True)
     sync()
-    tot = {"seconds": 0.0, "filter": 0.0, "fbytes": 0.0, "flaunch": 0, "cycles": 0, "maxres": 0.0}
+    tot = {"seconds": 0.0, "filter": 0.0, "fbytes": 0.0, "flaunch": 0, "cycles": 0, "maxres": 0.0,
+           "recoveries": 0, "rr_fallbacks": 0, "shifted": 0}
     t_start = time.perf_counter()
     for s in range(args.steps):
         st = b.solve()
@@ -139,27 +152,33 @@ def main():
         tot["flaunch"] += st.filter_launches
         tot["cycles"] = max(tot["cycles"], st.iterations)
         tot["maxres"] = max(tot["maxres"], st.max_residual)
+        tot["recoveries"] += st.recoveries
+        tot["rr_fallbacks"] += st.rr_cholqr_fallbacks
+        tot["shifted"] += st.shifted_cholqr
         assert st.converged == nts, "not every timeslice converged"
         print(f"[bench] step {s}: {st.seconds:.2f}s cycles={st.iterations} filter={st.seconds_filter:.2f}s "
-              f"dense={st.seconds_dense:.2f}s maxres={st.max_residual:.2e}", file=sys.stderr, flush=True)
+              f"dense={st.seconds_dense:.2f}s maxres={st.max_residual:.2e} rr_fallbacks={st.rr_cholqr_fallbacks} "
+              f"shifted={st.shifted_cholqr} recoveries={st.recoveries}", file=sys.stderr, flush=True)
     sync()
     wall = time.perf_counter() - t_start
+    if tot["recoveries"] > 0:
+        sys.exit(f"bench.py: {tot['recoveries']} recoveries (re-seeded blocks) in the timed steps")
     ms_per_step = wall / args.steps * 1e3
     value = nts * args.steps / wall
-    ev0, _, _, res0 = b.get(0, want_vectors=False)
-    lam_last = float(ev0[-1])
-    # dominant kernel: FP32 filter stencil, timed alone on the device (in the
-    # solve it shares the GPU with the other lane's dense FP64 work)
+    ev_batch = [b.get(t, want_vectors=False)[0] for t in range(nts)]
+    # dominant kernel: FP32 filter stencil, timed alone on the device
     ms_launch, bytes_launch = b.time_filter(reps=args.filter_reps)
     b.close()
     achieved = bytes_launch / (ms_launch * 1e-3)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                "frac": achieved / PEAK_HBM, "traffic": TRAFFIC_BYTES_PER_LAUNCH, "traffic_unit": "bytes/launch",
+                "frac": achieved / PEAK_HBM, "traffic": traffic_for_build(), "traffic_unit": "bytes/launch",
                 "kernel": (f"k_filter_step32x2y (block 128, y-row groups walking z-lines in runs of {st.filter_zrun})"
                            if st.filter_zrun else
                            f"k_filter_step32{'x2' if st.filter_block == 128 else ''} (block {st.filter_block})"),
                 "avg_launch_us": ms_launch * 1e3, "bytes_per_launch": bytes_launch,
                 "in_solve_GBps": tot["fbytes"] / tot["filter"] / 1e9 if tot["filter"] > 0 else None}
+    if roofline["traffic"] is None:
+        roofline["traffic_note"] = f"not measured for this build of {TRAFFIC['source']}"
 
     if not args.no_stream:
         # what a plain streaming kernel reaches for the timed step's access mix
@@ -168,9 +187,10 @@ def main():
         roofline["stream_1r1w_GBps"] = sc.get("1r1w_nt_GBps")
         roofline["frac_of_stream"] = achieved / 1e9 / sc["1r1w_nt_GBps"] if sc.get("1r1w_nt_GBps") else None
 
-    # plain FP64 drop-in on a few timeslices of the same field
-    lamC = 1.35 * abs(lam_last)
-    plain_times, plain_cycles = [], []
+    # plain FP64 drop-in on every timeslice of the same field, lambda_c from
+    # the free-field levels (no solve of the field needed to choose it)
+    lamC = plain_lambda_c(L, NEV)
+    plain_times, plain_cycles, plain_dev = [], [], 0.0
     for t in range(min(args.plain_ts, nts)):
         with lp.Tslice(L, DEG, LAMBDA_L, lamC, ts_config=cfg[t * vol:(t + 1) * vol]) as ts:
             ts.smearing_hyp(0.62, 0.62, 3)
@@ -179,11 +199,15 @@ def main():
             theta, _, pst = ts.eps_solve(NEV, tol=TOL, want_vectors=False)
             plain_times.append(time.perf_counter() - tp)
             plain_cycles.append(pst.restarts + 1)
+        ev = lp.recover_spectrum(theta, DEG, LAMBDA_L, lamC)
+        plain_dev = max(plain_dev, float(np.max(np.abs(ev - ev_batch[t]) / np.abs(ev_batch[t]))))
     plain_value = len(plain_times) / sum(plain_times) if plain_times else None
+    print(f"[bench] plain FP64: {len(plain_times)} timeslices, {plain_value:.2f} ts/s, batched-vs-plain max rel "
+          f"eigenvalue difference {plain_dev:.2e}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(cfg[:vol], lamC, int(np.median(plain_cycles)) if plain_cycles else 1)
+        cpu = cpu_baseline(cfg[:vol], lamC)
 
     out = {
         "metric": "eigensolved timeslices/s, 32^3 N_ev=220 res<=1e-10: mixed+batched vs plain FP64",
@@ -191,15 +215,19 @@ def main():
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32 filter + fp64 CholQR2/RR/lock", "data": "synthetic",
         "config": {"workload": f"{L}^3x{nts} eps={EPS} HYP 0.62/0.62/3, N_ev={NEV}, Chebyshev degree {DEG}, "
-                               f"one filter pass per cycle of degree <= 72 after one of degree {DEG}, vector block {st.filter_block}, "
-                               f"sweep strip {st.filter_strip}, search space {NSUB}, "
-                               f"residual <= {TOL}",
+                               f"one filter pass per cycle of degree <= 72 after one of degree {DEG}, vector block "
+                               f"{st.filter_block}, sweep strip {st.filter_strip}, search space {NSUB}, "
+                               f"residual <= {TOL}, ending with phase-fixed eigenpairs in HBM",
                    "lattice": f"{L}^3x{nts}", "nev": NEV, "deg": DEG, "tol": TOL, "parallelism": "none (1 GPU)"},
         "roofline": roofline,
         "plain_fp64": {"value": plain_value, "unit": "timeslices/s", "timeslices": len(plain_times),
-                       "lambda_c": lamC, "cycles": plain_cycles},
+                       "lambda_c": lamC, "cycles_max": max(plain_cycles) if plain_cycles else None,
+                       "seconds_per_timeslice": spread(plain_times),
+                       "max_rel_eigenvalue_difference_vs_batched": plain_dev},
         "speedup_vs_plain_fp64": (value / plain_value) if plain_value else None,
         "max_residual": tot["maxres"], "cycles": tot["cycles"],
+        "fallbacks": {"recoveries": tot["recoveries"], "rr_cholqr": tot["rr_fallbacks"],
+                      "shifted_cholqr": tot["shifted"]},
         "upload_map_hyp_seconds": t_hyp,
         "cpu_baseline": cpu,
     }
@@ -483,20 +511,122 @@ def device_sync():
     return sync
 
 
-def cpu_baseline(ts_config, lamC, cycles):
-    """Oracle Krylov-Schur (C, OpenMP) on one 32^3 timeslice for one restart
-    cycle (ncv = 440 Lanczos steps of T_16(B) with full re-orthogonalisation),
-    scaled by the cycle count of the identical algorithm on the GPU drop-in."""
+def plain_lambda_c(L, nev, factor=1.35):
+    """lambda_c of the plain drop-in: factor x the nev-th lowest level of -D
+    on the free field (every link 1: 4 sum_i sin^2(pi k_i / L), each level
+    three-fold).  HYP-smeared fields lie close above the free levels, so the
+    wanted nev eigenvalues stay below lambda_c without solving the field."""
+    s2 = 4 * np.sin(np.pi * np.arange(L) / L) ** 2
+    free = np.sort(np.repeat((s2[:, None, None] + s2[None, :, None] + s2[None, None, :]).ravel(), 3))
+    return factor * float(free[nev - 1])
+
+
+def spread(times):
+    if not times:
+        return None
+    return {"min": min(times), "median": float(np.median(times)), "max": max(times)}
+
+
+def traffic_for_build():
+    """The measured traffic constant, if the kernel source is the one measured."""
+    import hashlib
+    src = open(os.path.join(ROOT, TRAFFIC["source"]), "rb").read()
+    return TRAFFIC["bytes_per_launch"] if hashlib.sha256(src).hexdigest()[:16] == TRAFFIC["sha16"] else None
+
+
+def host_info():
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
+def _oracle_timeslice(ts_config):
     from oracle import oracle as O
-    U = O.hyp(O.map_timeslice(ts_config, L), L, 0.62, 0.62, 3)
-    t0 = time.perf_counter()
-    nconv, theta, V, stats = O.eps_solve(U, L, LAMBDA_L, lamC, DEG, NEV, tol=TOL, max_it=1)
-    dt = time.perf_counter() - t0
+    return O.hyp(O.map_timeslice(ts_config, L), L, 0.62, 0.62, 3)
+
+
+def cpu_baseline(ts_config, lamC, runs=3):
+    """The oracle's Krylov-Schur (C, OpenMP over the host cores) solving one
+    32^3 timeslice to 1e-10 with the plain drop-in's lambda_c, `runs` times;
+    the median solve time gives timeslices/s."""
+    from oracle import oracle as O
+    U = _oracle_timeslice(ts_config)
+    times, conv, matvecs = [], [], []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        nconv, theta, V, stats = O.eps_solve(U, L, LAMBDA_L, lamC, DEG, NEV, tol=TOL)
+        times.append(time.perf_counter() - t0)
+        conv.append(nconv)
+        matvecs.append(stats[1])
+    if min(conv) < NEV:
+        raise RuntimeError(f"cpu baseline: oracle solve converged {min(conv)} of {NEV} pairs")
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    per_ts = dt * max(1, cycles)
-    return {"value": 1.0 / per_ts, "unit": "timeslices/s", "cores": cores, "kind": "port",
-            "sample": f"one Krylov-Schur cycle ({stats[1]} applications of T_16(B), ncv=440) on one 32^3 timeslice "
-                      f"in {dt:.1f}s, scaled by the {cycles} cycle(s) the same algorithm needs on the GPU"}
+    med = float(np.median(times))
+    return {"value": 1.0 / med, "unit": "timeslices/s", "cores": cores, "kind": "port",
+            "sample": f"one 32^3 timeslice solved to {TOL} (N_ev {NEV}, T_{DEG}(B), ncv 440, {matvecs[0]} filtered "
+                      f"matvecs) with OpenMP over {cores} threads, median of {runs} solves "
+                      f"({', '.join(f'{t:.1f}' for t in times)} s); the one-timeslice-per-core plan over all 64 "
+                      f"timeslices is bench.py --cpu-full (not in the default run: minutes of host time)",
+            **host_info()}
+
+
+def cpu_worker(spec):
+    """One timeslice of --cpu-full: single-threaded oracle solve, times on stdout."""
+    from oracle import oracle as O
+    path, lam_c = spec.split(",")
+    cfg = np.load(path)
+    U = _oracle_timeslice(cfg)
+    t0 = time.perf_counter()
+    nconv, theta, V, stats = O.eps_solve(U, L, LAMBDA_L, float(lam_c), DEG, NEV, tol=TOL)
+    print(json.dumps({"seconds": time.perf_counter() - t0, "nconv": nconv, "matvecs": stats[1]}))
+
+
+def cpu_full(cores, runs, nts):
+    """The CPU baseline as planned: timeslices spread one per host core (the
+    reference's rank-per-timeslice split, ev_ts.cpp:96-126), each solved to
+    1e-10 by the single-threaded oracle; per run, `cores` timeslices start
+    together and the rate is cores / (time until the last finishes)."""
+    import subprocess
+    import tempfile
+    from oracle import lapheig_np as N
+    cores = cores or int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    lam_c = plain_lambda_c(L, NEV)
+    vol = 72 * L ** 3
+    cfg = N.synth_config(SEED, EPS, L, min(nts, cores))
+    tmp = tempfile.mkdtemp(prefix="laph_cpu_", dir=os.environ.get("TMPDIR", "/tmp"))
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    rates, per_ts = [], []
+    try:
+        paths = []
+        for t in range(min(nts, cores)):
+            p = os.path.join(tmp, f"ts{t}.npy")
+            np.save(p, cfg[t * vol:(t + 1) * vol])
+            paths.append(p)
+        for r in range(runs):
+            t0 = time.perf_counter()
+            procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker", f"{p},{lam_c}"],
+                                      stdout=subprocess.PIPE, text=True, env=env) for p in paths]
+            res = [json.loads(pr.communicate()[0]) for pr in procs]
+            wall = time.perf_counter() - t0
+            assert all(x["nconv"] >= NEV for x in res), res
+            rates.append(len(paths) / wall)
+            per_ts += [x["seconds"] for x in res]
+            print(f"[bench] cpu run {r}: {len(paths)} timeslices on {len(paths)} cores in {wall:.1f}s",
+                  file=sys.stderr, flush=True)
+    finally:
+        import shutil
+        shutil.rmtree(tmp, ignore_errors=True)
+    return {"value": float(np.median(rates)), "unit": "timeslices/s", "cores": len(paths), "kind": "port",
+            "sample": f"{len(paths)} timeslices of the 32^3x64 field, one per core (single-threaded oracle "
+                      f"Krylov-Schur to {TOL}, lambda_c {lam_c:.4f}), median rate of {runs} runs (process start and "
+                      "HYP included in each run's wall time)",
+            "rates": rates, "seconds_per_timeslice": spread(per_ts), **host_info()}
 
 
 if __name__ == "__main__":

@@ -242,6 +242,10 @@ typedef struct laph_batch_stats {
   double start_window;     /* window lower end of the first cycle from the start block: 3 x the
                               K-th level of the free Laplacian, at most 0.75 lambda_l
                               (LAPH_WINDOW0 fixes it) */
+  int rr_cholqr_fallbacks; /* Rayleigh-Ritz cycles whose Gram Cholesky failed, after which the
+                              block was orthonormalised by explicit CholQR2 */
+  int shifted_cholqr;      /* shifted Cholesky QR passes (a CholQR Gram that was not numerically
+                              positive definite) */
 } laph_batch_stats;
 
 int laph_batch_create(laph_batch** out, const laph_params* params, const laph_batch_opts* opts);
@@ -252,7 +256,9 @@ int laph_batch_set_timeslice(laph_batch* b, int ts, const double* ts_config, dou
                              double alpha_2, int iter);
 /* Upload already-smeared links of timeslice ts (host link layout). */
 int laph_batch_set_links(laph_batch* b, int ts, const double* links);
-/* Solve every timeslice of the batch. */
+/* Solve every timeslice of the batch; the solve ends with the nev eigenvectors
+ * of every timeslice phase-fixed (fix_phase, eigensystem.cpp:4-30) in device
+ * memory, and the phases recorded. */
 int laph_batch_solve(laph_batch* b, laph_batch_stats* stats);
 /* The three files of one solved timeslice straight from device memory
  * (ev_ts.cpp:304-318 for that timeslice): "<dir>/eigenvectors.%04d.%03d"

@@ -97,6 +97,8 @@ static int batch_alloc(laph_batch* b) {
   LAPH_HIP(hipMalloc(&b->first_blk, b->nts * sizeof(int)));
   LAPH_HIP(hipMalloc(&b->lock_blk, b->nts * sizeof(int)));
   LAPH_HIP(hipMalloc(&b->cn, nk * sizeof(double)));
+  LAPH_HIP(hipMalloc(&b->red, (size_t)red_chunks(b->n) * nk * sizeof(double)));
+  LAPH_HIP(hipMalloc(&b->phase, (size_t)b->nts * b->nev * sizeof(double)));
   LAPH_HIP(hipMalloc(&b->jsweeps, b->nts * sizeof(rocblas_int)));
   // defined contents for the width autotuner's first replays
   LAPH_HIP(hipMemset(b->coef, 0, (size_t)csteps * nk * sizeof(float4)));
@@ -153,11 +155,12 @@ static void batch_free(laph_batch* b) {
   for (void* h : b->pin)
     if (h) (void)hipHostFree(h);
   if (b->wrb) rocblas_destroy_handle(b->wrb);
-  for (void* q : {(void*)b->wV, (void*)b->wph, (void*)b->wG})
+  for (void* q : {(void*)b->wV, (void*)b->wG})
     if (q) (void)hipFree(q);
   void* bufs[] = {b->U64, b->U32, b->dec,  b->Unew, b->B[0], b->B[1],   b->B[2], b->R32,     b->P[0],
                   b->P[1], b->P[2], b->G,  b->G2,   b->P2,   b->G3,   b->mu,     b->E,    b->info,    b->nrm2,
-                  b->gam, b->coef, b->cab, b->first_blk, b->lock_blk, b->cn, b->coef2, b->jsweeps};
+                  b->gam, b->coef, b->cab, b->first_blk, b->lock_blk, b->cn, b->coef2, b->jsweeps,
+                  b->red, b->phase};
   for (void* q : bufs)
     if (q) (void)hipFree(q);
   if (b->st) (void)hipStreamDestroy(b->st);
@@ -189,6 +192,7 @@ struct View {
   size_t nk() const { return (size_t)ln->t0 * b->K; }
   size_t vo() const { return (size_t)ln->t0 * b->per_ts; }
   size_t lo() const { return (size_t)ln->t0 * b->V3 * 27; }
+  double* red() const { return b->red + (size_t)ln->t0 * red_chunks(b->n) * b->K; }
   int m() const { return ln->m > 0 ? ln->m : b->deg; }  // filter degree of the current cycle
 };
 
@@ -256,9 +260,10 @@ static int cholqr(const View& v, Act q, bool shifted = false) {
   if (!shifted && v.b->force_shift) LAPH_TRY(cholqr(v, q, true));  // LAPH_SHIFTED_CHOLQR (tests)
   // unit columns first: the filtered columns differ in norm by many orders
   // of magnitude (the span is unchanged by the scaling); then |Y_A|_2^2 <= a
-  LAPH_HIP(launch_normalize_columns(Y, v.b->cn + v.nk(), v.b->n, K, v.ln->nt, q.l, v.ln->st));
+  LAPH_HIP(launch_normalize_columns(Y, v.b->cn + v.nk(), v.red(), v.b->n, K, v.ln->nt, q.l, v.ln->st));
   LAPH_TRY(gram(v, q, Y, Y, G, 1.0));
   if (shifted) {
+    ++v.ln->shifted;
     const double a = q.a, u = 0x1p-53;
     const double sh = 11.0 * ((double)v.b->n * a + a * (a + 1.0)) * u * a;
     k_shift_diag<<<dim3((q.a + 255) / 256, v.ln->nt), 256, 0, v.ln->st>>>(G, q.a, K, sh);
@@ -367,6 +372,7 @@ static int rayleigh_ritz(const View& v, Act q, bool orthonormal, bool* chol_fail
     LAPH_HIP(hipGetLastError());
     LAPH_RB(rocsolver_zpotrf_strided_batched(rb, rocblas_fill_upper, q.a, zc(G2), K, kk, info, nt));
     if (dense_sync_info(v, "Rayleigh-Ritz (zpotrf)") != LAPH_OK) {
+      ++v.ln->rr_fallbacks;
       *chol_failed = true;
       return LAPH_OK;
     }
@@ -515,8 +521,8 @@ static int residuals_and_window(const View& v, std::vector<double>& lc, int* n_a
     // residual kernels derive the same window from the device copy of mu)
     lc[t] = std::min(b->h_mu[v.nk() + (size_t)t * K + K - 1], 0.75 * lamL);
   }
-  LAPH_HIP(launch_residual_norms(fb(b, b->R32, v.vo()), b->mu + v.nk(), lamL, b->nrm2 + v.nk(), b->n, K, nt, ln->lock,
-                                 b->bw, b->bf16, ln->st));
+  LAPH_HIP(launch_residual_norms(fb(b, b->R32, v.vo()), b->mu + v.nk(), lamL, b->nrm2 + v.nk(), v.red(), b->n, K, nt,
+                                 ln->lock, b->bw, b->bf16, ln->st));
   LAPH_HIP(hipMemcpyAsync(b->h_res.data() + v.nk(), b->nrm2 + v.nk(), (size_t)nt * K * sizeof(double),
                           hipMemcpyDeviceToHost, ln->st));
   LAPH_HIP(hipStreamSynchronize(ln->st));
@@ -965,6 +971,7 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
   ln->cycles = 0;
   ln->sv32 = ln->sv64 = ln->flaunch = 0;
   ln->recoveries = 0;
+  ln->rr_fallbacks = ln->shifted = 0;
   ln->fbytes = 0;
   ln->ms_filter = ln->ms_dense = 0;
   {
@@ -1132,7 +1139,6 @@ static int autotune_block_width(laph_batch* b) {
 // laph_batch_source_shape, allocated on first use.
 int output_buffers(laph_batch* b) {
   if (!b->wV) LAPH_HIP(hipMalloc(&b->wV, (size_t)b->n * b->nev * sizeof(double2)));
-  if (!b->wph) LAPH_HIP(hipMalloc(&b->wph, b->nev * sizeof(double)));
   return LAPH_OK;
 }
 
@@ -1158,6 +1164,12 @@ static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
   for (auto& t : th) t.join();
   for (auto& ln : b->lanes)
     if (ln.rc != LAPH_OK) return set_error(ln.rc, "%s", ln.msg.c_str());
+  // fix_phase (eigensystem.cpp:4-30) of every timeslice's nev eigenvectors,
+  // in place in B[0]: the solve ends with phase-fixed eigenpairs resident
+  LAPH_HIP(launch_fix_phases(b->B[0], b->phase, b->n, b->K, b->nev, b->nts, b->st));
+  LAPH_HIP(hipMemcpyAsync(b->h_phase.data(), b->phase, b->h_phase.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          b->st));
+  LAPH_HIP(hipStreamSynchronize(b->st));
   b->solved = true;
   int conv = 0, cycles = 0;
   double maxres = 0;
@@ -1172,6 +1184,8 @@ static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
     s.stencil64_vecs += ln.sv64;
     s.filter_launches += ln.flaunch;
     s.recoveries += ln.recoveries;
+    s.rr_cholqr_fallbacks += ln.rr_fallbacks;
+    s.shifted_cholqr += ln.shifted;
     s.filter_bytes += ln.fbytes;
     s.seconds_filter += ln.ms_filter * 1e-3;
     s.seconds_dense += ln.ms_dense * 1e-3;
@@ -1212,9 +1226,6 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
   K = (K + 63) / 64 * 64;
   LAPH_CHECK(K > o->nev && K <= n, LAPH_ERR_INVALID, "laph_batch_create: search space %d invalid for n=%d nev=%d", K,
              n, o->nev);
-  int ndev = 0;
-  LAPH_HIP(hipGetDeviceCount(&ndev));
-  LAPH_CHECK(p->device >= 0 && p->device < ndev, LAPH_ERR_INVALID, "laph_batch_create: no device %d", p->device);
   int strip = 0;  // LAPH_STRIP: fixed sweep strip width
   if (const char* e = std::getenv("LAPH_STRIP")) {
     strip = std::atoi(e);
@@ -1227,6 +1238,17 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     LAPH_CHECK(zrun == 0 || ((zrun == 1 || zrun == 2 || zrun == 4 || zrun == 8) && p->L % zrun == 0 && p->L % 4 == 0),
                LAPH_ERR_INVALID, "LAPH_ZRUN=%s: 0, or 1, 2, 4 or 8 dividing L=%d (a multiple of 4)", e, p->L);
   }
+  // automatic passes: one pass per cycle of the degree the window allows (up
+  // to 72); an explicit pass count: the batch's degree, up to that many passes
+  int fdeg_max = o->filter_passes > 0 ? 0 : 72;
+  if (const char* e = std::getenv("LAPH_FILTER_DEGREE")) {
+    fdeg_max = std::atoi(e);
+    LAPH_CHECK(fdeg_max == 0 || (fdeg_max >= 4 && fdeg_max <= 256), LAPH_ERR_INVALID,
+               "LAPH_FILTER_DEGREE=%s: 0, or a highest per-cycle degree in 4..256", e);
+  }
+  int ndev = 0;
+  LAPH_HIP(hipGetDeviceCount(&ndev));
+  LAPH_CHECK(p->device >= 0 && p->device < ndev, LAPH_ERR_INVALID, "laph_batch_create: no device %d", p->device);
   return laph::guarded([&]() -> int {
     laph_batch* b = new laph_batch();
     b->p = *p;
@@ -1248,11 +1270,9 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
       b->prod = std::atoi(e) != 0;
       b->qform = std::atoi(e) == 2;
     }
-    // automatic passes: one pass per cycle of the degree the window allows (up
-    // to 72); an explicit pass count: the batch's degree, up to that many passes
-    b->fdeg_max = o->filter_passes > 0 ? 0 : 72;
-    if (const char* e = std::getenv("LAPH_FILTER_DEGREE")) b->fdeg_max = std::max(0, std::min(256, std::atoi(e)));
+    b->fdeg_max = fdeg_max;
     if (b->fdeg_max > 0) b->prod = b->qform = true;  // one correction pass per cycle: product over q's roots
+    // cycle_degree returns a multiple of 4 in [4, fdeg_max]
     b->qrot.resize(std::max(p->deg, b->fdeg_max) + 1);
     for (int d = 2; d < (int)b->qrot.size(); ++d)
       if (d == p->deg || (b->fdeg_max > 0 && d % 4 == 0)) b->qrot[d] = laph::correction_root_rotations(d);
@@ -1276,6 +1296,7 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     b->nlock.assign(b->nts, 0);
     b->h_mu.assign((size_t)b->nts * K, 0.0);
     b->h_res.assign((size_t)b->nts * K, 0.0);
+    b->h_phase.assign((size_t)b->nts * o->nev, 0.0);
     auto fail = [&](int rc) {
       laph::batch_free(b);
       return rc;
@@ -1378,18 +1399,16 @@ int laph_batch_get(laph_batch* b, int ts, double* evals, double* V_fix, double* 
   for (int j = 0; j < nev; ++j) evals[j] = -b->h_mu[(size_t)ts * K + j];
   if (residuals)
     for (int j = 0; j < nev; ++j) residuals[j] = std::sqrt(b->h_res[(size_t)ts * K + j]);
-  if (V_fix || phase) {
+  if (phase)
+    for (int j = 0; j < nev; ++j) phase[j] = b->h_phase[(size_t)ts * nev + j];
+  if (V_fix) {  // the solve left them phase-fixed in B[0]: only the layout changes
     double2* dV = nullptr;
-    double* dph = nullptr;
     LAPH_HIP(hipMalloc(&dV, (size_t)b->n * nev * sizeof(double2)));
-    hipError_t e = hipMalloc(&dph, nev * sizeof(double));
-    if (e == hipSuccess) e = laph::launch_extract_phase(b->B[0] + (size_t)ts * b->per_ts, dV, dph, b->n, K, nev, b->st);
-    if (e == hipSuccess && V_fix)
+    hipError_t e = laph::launch_extract_cols(b->B[0] + (size_t)ts * b->per_ts, dV, b->n, K, nev, b->st);
+    if (e == hipSuccess)
       e = hipMemcpyAsync(V_fix, dV, (size_t)b->n * nev * sizeof(double2), hipMemcpyDeviceToHost, b->st);
-    if (e == hipSuccess && phase) e = hipMemcpyAsync(phase, dph, nev * sizeof(double), hipMemcpyDeviceToHost, b->st);
     if (e == hipSuccess) e = hipStreamSynchronize(b->st);
     (void)hipFree(dV);
-    if (dph) (void)hipFree(dph);
     LAPH_HIP(e);
   }
   return LAPH_OK;
@@ -1413,15 +1432,16 @@ int laph_batch_write(laph_batch* b, int ts, const char* dir, int config, int t, 
         LAPH_HIP(hipEventCreateWithFlags(&b->wev[i], hipEventDisableTiming));
       }
     }
-    // phase-fixed eigenvectors (eigensystem.cpp:4-30) and their Gram V^H V
-    LAPH_HIP(laph::launch_extract_phase(b->B[0] + (size_t)ts * b->per_ts, b->wV, b->wph, n, K, nev, b->st));
+    // phase-fixed eigenvectors (fixed in place by the solve, eigensystem.cpp:4-30)
+    // in the file layout, and their Gram V^H V
+    LAPH_HIP(laph::launch_extract_cols(b->B[0] + (size_t)ts * b->per_ts, b->wV, n, K, nev, b->st));
     const laph::cplx one(1.0, 0.0), zero(0.0, 0.0);
     LAPH_RB(rocblas_zgemm(b->wrb, rocblas_operation_conjugate_transpose, rocblas_operation_none, nev, nev, n, &one,
                           laph::zc(b->wV), n, laph::zc(b->wV), n, &zero, laph::zc(b->wG), nev));
     std::vector<double2> G((size_t)nev * nev);
     std::vector<double> ph(nev), ev(nev);
     LAPH_HIP(hipMemcpyAsync(G.data(), b->wG, G.size() * sizeof(double2), hipMemcpyDeviceToHost, b->st));
-    LAPH_HIP(hipMemcpyAsync(ph.data(), b->wph, nev * sizeof(double), hipMemcpyDeviceToHost, b->st));
+    std::copy(b->h_phase.begin() + (size_t)ts * nev, b->h_phase.begin() + (size_t)(ts + 1) * nev, ph.begin());
     LAPH_HIP(hipStreamSynchronize(b->st));
     double tr[2] = {0, 0}, sm[2] = {0, 0};
     for (int i = 0; i < nev; ++i) {

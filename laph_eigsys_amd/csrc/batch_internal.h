@@ -40,6 +40,8 @@ struct laph_lane {
   std::string msg;  // error message raised on the lane's thread
   long long sv32 = 0, sv64 = 0, flaunch = 0;
   int recoveries = 0;  // timeslice blocks re-seeded after a failed orthonormalisation
+  int rr_fallbacks = 0;  // Rayleigh-Ritz Gram Cholesky failures (block then orthonormalised by CholQR2)
+  int shifted = 0;       // shifted Cholesky QR passes (plain CholQR Gram not numerically definite)
   double fbytes = 0;
   float ms_filter = 0, ms_dense = 0;
 };
@@ -71,6 +73,8 @@ struct laph_batch {
   rocblas_int* info = nullptr;
   double* nrm2 = nullptr;    // [nts][K]
   double* cn = nullptr;      // [nts][K] column norms for CholQR scaling
+  double* red = nullptr;     // [nts][red_chunks(n)][K] partial column sums (deterministic reductions)
+  double* phase = nullptr;   // [nts][nev] phases of the phase-fixed eigenvectors (fix_phase)
   double* gam = nullptr;     // [nts][K]
   float4* coef = nullptr;    // [deg-1][nts][K]
   float4* coef2 = nullptr;   // [deg - 1 + LAPH_MAX_PASSES - 1][nts][K] plain passes
@@ -79,7 +83,7 @@ struct laph_batch {
   int* lock_blk = nullptr;   // [nts] first active 64-column block of the dense steps
   // host mirrors, written by the lanes on disjoint ranges
   std::vector<int> nlock;
-  std::vector<double> h_mu, h_res;
+  std::vector<double> h_mu, h_res, h_phase;
   std::vector<laph_lane> lanes;
   bool solved = false;
   int eig_mode = 0;          // Rayleigh-Ritz eigensolver: 0 zheevd, 1 zheevdj, 2 zheevj (LAPH_EIG)
@@ -99,10 +103,9 @@ struct laph_batch {
   bool tuned = false;        // filter layout chosen (first solve)
   int poison_ts = -1;  // LAPH_TEST_POISON: NaN into one row of this timeslice's active block at cycle 2 (tests)
   int qr_cycles = 0;  // cycles (and the start block) orthonormalised by explicit CholQR2 (LAPH_QR_CYCLES)
-  // laph_batch_write: phase-fixed copy, phases, V^H V, rocBLAS handle on st,
+  // laph_batch_write: phase-fixed copy in the file layout, V^H V, rocBLAS handle on st,
   // two pinned staging chunks and their events (created on first use)
   double2* wV = nullptr;
-  double* wph = nullptr;
   double2* wG = nullptr;
   rocblas_handle wrb = nullptr;
   void* pin[2] = {nullptr, nullptr};
@@ -129,6 +132,6 @@ double free_level(int L, int K);
 // z-sites per wavefront of the y-row work-group stencils for the batch's sweep
 // strip (they need a strip width that is a multiple of 4); 0: one site each.
 inline int stencil_zrun(const laph_batch* b) { return b->strip % 4 == 0 ? b->zrun : 0; }
-// Allocates laph_batch::wV / wph on first use.
+// Allocates laph_batch::wV on first use.
 int output_buffers(laph_batch* b);
 }  // namespace laph

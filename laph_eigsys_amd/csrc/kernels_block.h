@@ -35,20 +35,30 @@ hipError_t launch_lap_block64(const double2* U64, const double2* v, double2* out
 hipError_t launch_lap_residual64(const double2* U64, const double2* X, const double* mu, double lamL, void* r,
                                  int L, int K, int nts, const int* first_blk, int W, int bw, int bf16, int zrun,
                                  hipStream_t st);
+// Column sums over a block's rows go through per-chunk partials of kRedRows
+// rows, added in chunk order (deterministic: no atomics); `red` holds
+// nts * red_chunks(n) * K doubles.
+constexpr int kRedRows = 512;
+inline int red_chunks(int n) { return (n + kRedRows - 1) / kRedRows; }
 // nrm2 = column sums of |D X + X diag(mu)|^2 from that r, for the columns
 // >= first_col (a multiple of 64).
-hipError_t launch_residual_norms(const void* r, const double* mu, double lamL, double* nrm2, int n, int K, int nts,
-                                 int first_col, int bw, int bf16, hipStream_t st);
-// Column-major, phase-fixed copy of the first nev columns of one timeslice.
+hipError_t launch_residual_norms(const void* r, const double* mu, double lamL, double* nrm2, double* red, int n,
+                                 int K, int nts, int first_col, int bw, int bf16, hipStream_t st);
+// fix_phase (eigensystem.cpp:4-30) in place on the first nev columns of every
+// timeslice of a [ts][n][K] block: phase[ts][col] = arg X[ts][0][col], then
+// each column times e^{-i phase}.
+hipError_t launch_fix_phases(double2* X, double* phase, int n, int K, int nev, int nts, hipStream_t st);
+// V (column-major n x nev, the eigenvector-file layout) = the first nev
+// columns of one timeslice's [n][K] block.
+hipError_t launch_extract_cols(const double2* X, double2* V, int n, int K, int nev, hipStream_t st);
 // psi(r) = sum_x |V_{x+r} V_x^H|_F for r in {0, 2, ..}^3 below L, V in the
 // eigenvector-file layout (n x nev column-major); psi has ((L+1)/2)^3 entries
 // in the order r1, r2, r3 (r3 fastest).
 hipError_t launch_source_shape(const double2* V, int L, int nev, double* psi, hipStream_t st);
-hipError_t launch_extract_phase(const double2* X, double2* V, double* phase, int n, int K, int nev,
-                                hipStream_t st);
 // Scales every column >= first_col (a multiple of 64) of a [ts][n][K] block to
-// unit norm; tmp holds nts*K doubles.
-hipError_t launch_normalize_columns(double2* Y, double* tmp, int n, int K, int nts, int first_col, hipStream_t st);
+// unit norm; tmp holds nts*K doubles (the squared norms), red the partials.
+hipError_t launch_normalize_columns(double2* Y, double* tmp, double* red, int n, int K, int nts, int first_col,
+                                    hipStream_t st);
 hipError_t launch_convert_links32(const double2* U, float2* U32, size_t n, hipStream_t st);
 hipError_t launch_random_block(double2* X, size_t total, unsigned long long seed, hipStream_t st);
 // Gaussian columns [first_col, K) of the timeslices with flag[ts] != 0.

@@ -815,17 +815,20 @@ __global__ __launch_bounds__(256) void k_lap64y(const double2* __restrict__ U64,
   }
 }
 
-// nrm2[ts][col] += |R|^2 summed over the rows of one chunk, from r = c_t R in
-// the filter storage, for the columns >= first_col.  grid (row chunks,
-// active 64-column groups, ts); each thread sums the rows of one column.
+// part[ts][chunk][col] = |R|^2 summed over the kRedRows rows of one chunk,
+// from r = c_t R in the filter storage, for the columns >= first_col.  grid
+// (row chunks, active 64-column groups, ts); each thread sums every 4th row of
+// one column, then the block adds its four row lanes in a fixed order.  The
+// chunk partials are summed in chunk order by k_sum_chunks: no atomics, so a
+// repeated solve reproduces every bit.
 __global__ __launch_bounds__(256) void k_residual_norms(const void* __restrict__ rbuf,
                                                          const double* __restrict__ mu, double lamL,
-                                                         double* __restrict__ nrm2, int n, int K,
-                                                         int rows_per_block, int first_col, int bw, int bf16) {
+                                                         double* __restrict__ red, int n, int K,
+                                                         int first_col, int bw, int bf16) {
   const int ts = blockIdx.z;
   const int col = first_col + blockIdx.y * 64 + (threadIdx.x & 63);
-  const int r0 = blockIdx.x * rows_per_block + (threadIdx.x >> 6);
-  const int r1 = min(n, (int)(blockIdx.x + 1) * rows_per_block);
+  const int r0 = blockIdx.x * kRedRows + (threadIdx.x >> 6);
+  const int r1 = min(n, (int)(blockIdx.x + 1) * kRedRows);
   const double c = window_scale(mu + (size_t)ts * K, K, lamL);
   const size_t base = ((size_t)ts * (K / bw) + col / bw) * n * bw + col % bw;
   double acc = 0.0;
@@ -837,33 +840,57 @@ __global__ __launch_bounds__(256) void k_residual_norms(const void* __restrict__
   __shared__ double part[256];
   part[threadIdx.x] = acc / (c * c);
   __syncthreads();
-  if (threadIdx.x < 64) {
-    const double t = part[threadIdx.x] + part[threadIdx.x + 64] + part[threadIdx.x + 128] + part[threadIdx.x + 192];
-    atomicAdd(&nrm2[(size_t)ts * K + col], t);
-  }
+  if (threadIdx.x < 64)
+    red[((size_t)ts * gridDim.x + blockIdx.x) * K + col] =
+        part[threadIdx.x] + part[threadIdx.x + 64] + part[threadIdx.x + 128] + part[threadIdx.x + 192];
 }
 
-// Zero nrm2[ts][col] for col >= first_col.
-__global__ void k_zero_cols(double* __restrict__ p, int nts, int K, int first_col) {
+// out[ts][col] = sum over chunk c = 0 .. nchunk-1 (in that order) of
+// red[ts][c][col], for col >= first_col.
+__global__ void k_sum_chunks(const double* __restrict__ red, double* __restrict__ out, int nts, int K, int first_col,
+                             int nchunk) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int w = K - first_col;
   if (i >= nts * w) return;
-  p[(size_t)(i / w) * K + first_col + i % w] = 0.0;
+  const int ts = i / w, col = first_col + i % w;
+  const double* p = red + (size_t)ts * nchunk * K + col;
+  double acc = 0.0;
+  for (int c = 0; c < nchunk; ++c) acc += p[(size_t)c * K];
+  out[(size_t)ts * K + col] = acc;
 }
 
-// Output: V[col][row] (column-major n x nev) = e^{-i phi_col} X[row][col],
-// phi_col = arg X[0][col] (fix_phase, eigensystem.cpp:4-30).
-__global__ void k_extract_phase(const double2* __restrict__ X, double2* __restrict__ V,
-                                double* __restrict__ phase, int n, int K, int nev) {
-  const int col = blockIdx.y;
-  const double2 x0 = X[col];
-  const double ph = atan2(x0.y, x0.x);
+// Phases phi[ts][col] = arg X[ts][0][col] of the first nev columns
+// (eigensystem.cpp:21), then X[ts][row][col] *= e^{-i phi} in place
+// (eigensystem.cpp:22-27): two kernels, so row 0 is read before it changes.
+__global__ void k_phases(const double2* __restrict__ X, double* __restrict__ phase, size_t per_ts, int K, int nev,
+                         int nts) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nts * nev) return;
+  const int ts = i / nev, col = i % nev;
+  const double2 x0 = X[(size_t)ts * per_ts + col];
+  phase[i] = atan2(x0.y, x0.x);
+}
+
+__global__ void k_apply_phases(double2* __restrict__ X, const double* __restrict__ phase, size_t per_ts, int K,
+                               int nev) {
+  const int ts = blockIdx.y;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= per_ts) return;
+  const int col = (int)(i % K);
+  if (col >= nev) return;
+  const double ph = phase[(size_t)ts * nev + col];
   const double cr = cos(ph), si = -sin(ph);
-  if (blockIdx.x == 0 && threadIdx.x == 0) phase[col] = ph;
-  for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < n; row += gridDim.x * blockDim.x) {
-    const double2 x = X[(size_t)row * K + col];
-    V[(size_t)col * n + row] = make_double2(x.x * cr - x.y * si, x.x * si + x.y * cr);
-  }
+  const size_t e = (size_t)ts * per_ts + i;
+  const double2 x = X[e];
+  X[e] = make_double2(x.x * cr - x.y * si, x.x * si + x.y * cr);
+}
+
+// V[col][row] (column-major n x nev, the eigenvector-file layout) =
+// X[row][col] for the first nev columns of one timeslice.
+__global__ void k_extract_cols(const double2* __restrict__ X, double2* __restrict__ V, int n, int K) {
+  const int col = blockIdx.y;
+  for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < n; row += gridDim.x * blockDim.x)
+    V[(size_t)col * n + row] = X[(size_t)row * K + col];
 }
 
 __global__ void k_convert_links32(const double2* __restrict__ U, float2* __restrict__ U32, size_t n) {
@@ -1044,20 +1071,25 @@ hipError_t launch_lap_residual64(const double2* U64, const double2* X, const dou
   return hipGetLastError();
 }
 
-hipError_t launch_residual_norms(const void* r, const double* mu, double lamL, double* nrm2, int n, int K, int nts,
-                                 int first_col, int bw, int bf16, hipStream_t st) {
+hipError_t launch_residual_norms(const void* r, const double* mu, double lamL, double* nrm2, double* red, int n,
+                                 int K, int nts, int first_col, int bw, int bf16, hipStream_t st) {
   const int w = K - first_col;
-  k_zero_cols<<<(nts * w + 255) / 256, 256, 0, st>>>(nrm2, nts, K, first_col);
-  const int rows = 256;
-  dim3 g((n + rows - 1) / rows, w / 64, nts);
-  k_residual_norms<<<g, 256, 0, st>>>(r, mu, lamL, nrm2, n, K, rows, first_col, bw, bf16);
+  const int nchunk = red_chunks(n);
+  dim3 g(nchunk, w / 64, nts);
+  k_residual_norms<<<g, 256, 0, st>>>(r, mu, lamL, red, n, K, first_col, bw, bf16);
+  k_sum_chunks<<<(nts * w + 255) / 256, 256, 0, st>>>(red, nrm2, nts, K, first_col, nchunk);
   return hipGetLastError();
 }
 
-hipError_t launch_extract_phase(const double2* X, double2* V, double* phase, int n, int K, int nev,
-                                hipStream_t st) {
-  dim3 g(64, nev);
-  k_extract_phase<<<g, 256, 0, st>>>(X, V, phase, n, K, nev);
+hipError_t launch_fix_phases(double2* X, double* phase, int n, int K, int nev, int nts, hipStream_t st) {
+  const size_t per_ts = (size_t)n * K;
+  k_phases<<<(nts * nev + 255) / 256, 256, 0, st>>>(X, phase, per_ts, K, nev, nts);
+  k_apply_phases<<<dim3((unsigned)((per_ts + 255) / 256), nts), 256, 0, st>>>(X, phase, per_ts, K, nev);
+  return hipGetLastError();
+}
+
+hipError_t launch_extract_cols(const double2* X, double2* V, int n, int K, int nev, hipStream_t st) {
+  k_extract_cols<<<dim3(64, nev), 256, 0, st>>>(X, V, n, K);
   return hipGetLastError();
 }
 
@@ -1082,12 +1114,14 @@ hipError_t launch_random_block(double2* X, size_t total, unsigned long long seed
 
 namespace laph {
 
-// nrm[ts][col] = sum_rows |Y[ts][row][col]|^2 for col >= first_col (zeroed first).
-__global__ __launch_bounds__(256) void k_colnorm2_blk(const double2* __restrict__ Y, double* __restrict__ nrm, int n,
+// red[ts][chunk][col] = sum over the kRedRows rows of one chunk of
+// |Y[ts][row][col]|^2 for col >= first_col (summed in chunk order by
+// k_sum_chunks, as the residual norms).
+__global__ __launch_bounds__(256) void k_colnorm2_blk(const double2* __restrict__ Y, double* __restrict__ red, int n,
                                                        int K, int first_col) {
   const int ts = blockIdx.z;
   const int col = first_col + blockIdx.y * 64 + (threadIdx.x & 63);
-  const int r0 = blockIdx.x * 256 + (threadIdx.x >> 6), r1 = min(n, (int)(blockIdx.x + 1) * 256);
+  const int r0 = blockIdx.x * kRedRows + (threadIdx.x >> 6), r1 = min(n, (int)(blockIdx.x + 1) * kRedRows);
   const size_t off = (size_t)ts * n * K;
   double acc = 0.0;
   for (int r = r0; r < r1; r += 4) {
@@ -1098,8 +1132,8 @@ __global__ __launch_bounds__(256) void k_colnorm2_blk(const double2* __restrict_
   part[threadIdx.x] = acc;
   __syncthreads();
   if (threadIdx.x < 64)
-    atomicAdd(&nrm[(size_t)ts * K + col],
-              part[threadIdx.x] + part[threadIdx.x + 64] + part[threadIdx.x + 128] + part[threadIdx.x + 192]);
+    red[((size_t)ts * gridDim.x + blockIdx.x) * K + col] =
+        part[threadIdx.x] + part[threadIdx.x + 64] + part[threadIdx.x + 128] + part[threadIdx.x + 192];
 }
 
 // Y[ts][row][col] /= sqrt(nrm[ts][col]) for col >= first_col.
@@ -1116,11 +1150,13 @@ __global__ void k_scale_cols(double2* __restrict__ Y, const double* __restrict__
   Y[(size_t)ts * per_ts + i] = make_double2(v.x * s, v.y * s);
 }
 
-hipError_t launch_normalize_columns(double2* Y, double* tmp, int n, int K, int nts, int first_col, hipStream_t st) {
+hipError_t launch_normalize_columns(double2* Y, double* tmp, double* red, int n, int K, int nts, int first_col,
+                                    hipStream_t st) {
   const int w = K - first_col;
-  k_zero_cols<<<(nts * w + 255) / 256, 256, 0, st>>>(tmp, nts, K, first_col);
-  dim3 g((n + 255) / 256, w / 64, nts);
-  k_colnorm2_blk<<<g, 256, 0, st>>>(Y, tmp, n, K, first_col);
+  const int nchunk = red_chunks(n);
+  dim3 g(nchunk, w / 64, nts);
+  k_colnorm2_blk<<<g, 256, 0, st>>>(Y, red, n, K, first_col);
+  k_sum_chunks<<<(nts * w + 255) / 256, 256, 0, st>>>(red, tmp, nts, K, first_col, nchunk);
   const size_t per_ts = (size_t)n * K;
   dim3 gs((unsigned)((per_ts + 255) / 256), nts);
   k_scale_cols<<<gs, 256, 0, st>>>(Y, tmp, per_ts, K, first_col);

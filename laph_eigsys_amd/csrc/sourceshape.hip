@@ -105,7 +105,7 @@ int laph_batch_source_shape(laph_batch* b, int ts, double* psi) {
   return laph::guarded([&]() -> int {
     LAPH_TRY(laph::output_buffers(b));
     const int L = b->p.L, half = (L + 1) / 2, nr = half * half * half;
-    LAPH_HIP(laph::launch_extract_phase(b->B[0] + (size_t)ts * b->per_ts, b->wV, b->wph, b->n, b->K, b->nev, b->st));
+    LAPH_HIP(laph::launch_extract_cols(b->B[0] + (size_t)ts * b->per_ts, b->wV, b->n, b->K, b->nev, b->st));
     double* dpsi = nullptr;
     LAPH_HIP(hipMalloc(&dpsi, nr * sizeof(double)));
     hipError_t e = laph::launch_source_shape(b->wV, L, b->nev, dpsi, b->st);

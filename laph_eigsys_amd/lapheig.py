@@ -63,7 +63,8 @@ class laph_batch_stats(ctypes.Structure):
                 ("seconds_filter", ctypes.c_double), ("seconds_dense", ctypes.c_double),
                 ("filter_launches", ctypes.c_longlong), ("filter_bytes", ctypes.c_double),
                 ("filter_block", ctypes.c_int), ("filter_strip", ctypes.c_int), ("recoveries", ctypes.c_int),
-                ("filter_zrun", ctypes.c_int), ("start_window", ctypes.c_double)]
+                ("filter_zrun", ctypes.c_int), ("start_window", ctypes.c_double),
+                ("rr_cholqr_fallbacks", ctypes.c_int), ("shifted_cholqr", ctypes.c_int)]
 
 
 # (name, argtypes) of every entry point declared in include/laph.h

@@ -331,3 +331,31 @@ def test_recovery_from_a_non_finite_block(lp, gpu, monkeypatch):
         ev, V, ph, res = b.get(t)
         check_pairs(ev, V, res, N.dense_D(U.reshape(L, L, L, 3, 3, 3)), nev, tol)
     b.close()
+
+
+def test_batch_solved_twice(lp, gpu):
+    """One batch solved twice in a row (the second solve reuses the captured
+    filter graphs, the layout chosen by the first and every work buffer):
+    both spectra match dense diagonalisation, and the second solve repeats
+    the first bit for bit -- the column-norm reductions add fixed-order
+    partials, so nothing depends on the order atomics would land in."""
+    L, nts, nev, tol = 8, 3, 24, 1e-10
+    data = smeared(L, 31, 0.4, nts)
+    b = lp.Batch(L, 8, 12.0, 0.0, nts, nev, nsub=64, tol=tol, seed=5, device=gpu)
+    for t, (ts, _) in enumerate(data):
+        b.set_timeslice(t, ts, 0.62, 0.62, 3)
+    runs = []
+    for rep in range(2):
+        st = b.solve()
+        print(f"solve {rep}: cycles {st.iterations} recoveries {st.recoveries} rr fallbacks "
+              f"{st.rr_cholqr_fallbacks} shifted {st.shifted_cholqr}")
+        assert st.converged == nts and st.recoveries == 0
+        runs.append([b.get(t) for t in range(nts)])
+    for t, (_, U) in enumerate(data):
+        D = N.dense_D(U.reshape(L, L, L, 3, 3, 3))
+        for r in runs:
+            ev, V, ph, res = r[t]
+            check_pairs(ev, V, res, D, nev, tol)
+        for a, c in zip(runs[0][t], runs[1][t]):
+            np.testing.assert_array_equal(a, c)
+    b.close()

@@ -167,3 +167,15 @@ def test_batch_rejects_bad_arguments(L, kw):
         L.Batch(a["L"], a["deg"], a["lambda_l"], 0.0, a["nts"], a["nev"], nsub=a["nsub"], tol=a["tol"],
                 filter_passes=a["filter_passes"], filter_storage=a["filter_storage"])
     assert e.value.code == -1 and "laph_batch_create" in str(e.value)
+
+
+@pytest.mark.parametrize("value", ["1", "2", "3", "300"])
+def test_batch_rejects_filter_degree_below_4(L, monkeypatch, value):
+    """LAPH_FILTER_DEGREE must be 0 or a highest per-cycle degree in 4..256
+    (the per-cycle degree is a multiple of 4, at least 4, and the root and
+    coefficient tables are sized by it): other values are rejected before
+    any device work."""
+    monkeypatch.setenv("LAPH_FILTER_DEGREE", value)
+    with pytest.raises(L.LaphError) as e:
+        L.Batch(8, 8, 12.0, 0.0, 2, 10)
+    assert e.value.code == -1 and "LAPH_FILTER_DEGREE" in str(e.value)
