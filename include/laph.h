@@ -85,6 +85,23 @@ int laph_shell_get_diagonal(void* ctx, double* diag);
 /* Device-pointer variant of laph_shell_mult (n complex on the context's
  * device; completes before returning). */
 int laph_shell_mult_device(laph_ctx* ctx, const double* dx, double* dy);
+/* Variant of laph_shell_mult_device that allocates nothing and needs no
+ * context.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  y = T_DEG(B) x with the parameters of
+ * *params (L, DEG, LAM_L, LAM_C, device); d_links: the smeared links on the
+ * device in the host link layout above (V3*3*9 complex); dx, dy: device
+ * vectors of n = 3*V3 complex, distinct.  The kernels are enqueued on
+ * `stream` (a hipStream_t, NULL = the null stream) and the call returns
+ * without waiting; dy is also used as one of the three recurrence buffers.
+ * The other two live in a fixed device array of the library (2 * 3 * 48^3
+ * complex doubles per device, 10.6 MB, held until the process ends): L > 48
+ * is rejected with LAPH_ERR_INVALID.  Every call on a device shares that
+ * array, so calls that overlap on one device (other streams, other host
+ * threads) race; calls in one stream are ordered and safe.  Results are
+ * bitwise equal to laph_shell_mult_device's. */
+int laph_shell_mult_noalloc(const laph_params* params, const double* d_links, const double* dx, double* dy,
+                            void* stream);
 /* y = D x (tv2 with c = 1, a = 0; shell_matop.cpp:68-77), host arrays. */
 int laph_laplacian(laph_ctx* ctx, const double* x, double* y);
 

@@ -1,11 +1,39 @@
 // C-ABI entry points of the per-timeslice context: link upload
 // (map_timeslice_to_eigen), HYP, and the MatShell callbacks.
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "ctx.h"
 
 namespace laph {
+
+// Recurrence buffers of laph_shell_mult_noalloc: two vectors of up to
+// 3 * 48^3 complex doubles (10.6 MB), held by every device the library runs
+// on until the process ends.  Host code never passes `scratch` itself to a
+// kernel: its device address comes from hipGetSymbolAddress, per device.
+constexpr int kNoallocMaxL = 48;
+constexpr size_t kNoallocVec = 3ull * kNoallocMaxL * kNoallocMaxL * kNoallocMaxL;
+static __device__ __align__(256) double2 scratch[2 * kNoallocVec];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double2* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on device `dev` (current), looked up on the
+// first call on that device.
+static int noalloc_scratch(int dev, double2** out) {
+  LAPH_CHECK(dev >= 0 && dev < kMaxDevices, LAPH_ERR_INVALID, "device %d beyond the scratch table", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    LAPH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    LAPH_CHECK(p != nullptr, LAPH_ERR_HIP, "hipGetSymbolAddress returned no address on device %d", dev);
+    g_scratch_ptr[dev] = static_cast<double2*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return LAPH_OK;
+}
 
 int ensure_packed(laph_ctx* ctx) {
   if (!ctx->links_dirty) return LAPH_OK;
@@ -209,6 +237,48 @@ int laph_shell_mult_device(laph_ctx* ctx, const double* dx, double* dy) {
   LAPH_HIP(hipSetDevice(ctx->p.device));
   LAPH_TRY(laph::cheb_apply(ctx, reinterpret_cast<const double2*>(dx), reinterpret_cast<double2*>(dy)));
   LAPH_HIP(hipStreamSynchronize(ctx->stream));
+  return LAPH_OK;
+}
+
+int laph_shell_mult_noalloc(const laph_params* p, const double* d_links, const double* dx, double* dy,
+                            void* stream) {
+  LAPH_CHECK(p && d_links && dx && dy && dx != dy && (const double*)dy != d_links, LAPH_ERR_INVALID,
+             "laph_shell_mult_noalloc: null or aliased arguments");
+  LAPH_CHECK(p->L >= 3 && p->L <= laph::kNoallocMaxL, LAPH_ERR_INVALID,
+             "laph_shell_mult_noalloc: L=%d outside [3, %d] (the fixed scratch holds two vectors of 3*%d^3)", p->L,
+             laph::kNoallocMaxL, laph::kNoallocMaxL);
+  LAPH_CHECK(p->deg >= 1, LAPH_ERR_INVALID, "laph_shell_mult_noalloc: DEG=%d < 1", p->deg);
+  LAPH_CHECK(p->lambda_l > p->lambda_c, LAPH_ERR_INVALID,
+             "laph_shell_mult_noalloc: lambda_l=%g must exceed lambda_c=%g", p->lambda_l, p->lambda_c);
+  int ndev = 0;
+  LAPH_HIP(hipGetDeviceCount(&ndev));
+  LAPH_CHECK(p->device >= 0 && p->device < ndev, LAPH_ERR_INVALID, "laph_shell_mult_noalloc: no device %d",
+             p->device);
+  LAPH_HIP(hipSetDevice(p->device));
+  double2* t = nullptr;
+  LAPH_TRY(laph::noalloc_scratch(p->device, &t));
+  const int L = p->L, deg = p->deg;
+  const size_t n = 3ull * L * L * L;
+  const double c = 2.0 / (p->lambda_l - p->lambda_c), a = 1.0 + c * p->lambda_c;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const double2* U = reinterpret_cast<const double2*>(d_links);
+  const double2* x = reinterpret_cast<const double2*>(dx);
+  double2* y = reinterpret_cast<double2*>(dy);
+  // T_1 = B x, T_k = 2 B T_{k-1} - T_{k-2} (shell_matop.cpp:150-166) with the
+  // outputs rotating over {y, t0, t1} so that step k writes buf[(deg-k) % 3]:
+  // the last step writes y, and no step's output is its input or T_{k-2}.
+  double2* buf[3] = {y, t, t + n};
+  const double2* old = nullptr;
+  const double2* cur = x;
+  for (int k = 1; k <= deg; ++k) {
+    double2* out = buf[(deg - k) % 3];
+    if (k == 1)
+      LAPH_HIP(laph::launch_cheb_step64_aos(U, cur, nullptr, out, L, c, a, 1.0, 0.0, st));
+    else
+      LAPH_HIP(laph::launch_cheb_step64_aos(U, cur, old, out, L, c, a, 2.0, -1.0, st));
+    old = cur;
+    cur = out;
+  }
   return LAPH_OK;
 }
 

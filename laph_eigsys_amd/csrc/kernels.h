@@ -25,5 +25,10 @@ hipError_t launch_pack_links32(const double2* U, float2* Us, int L, hipStream_t 
 hipError_t launch_cheb_step64(const double2* Us, const double2* in, const double2* old,
                               double2* out, int L, double c, double a, double alpha,
                               double beta, hipStream_t st);
+// The same step reading the links in the AoS host layout U[(s*3+dir)*9 + e]
+// (the arithmetic is identical, so the results are bitwise equal).
+hipError_t launch_cheb_step64_aos(const double2* U, const double2* in, const double2* old,
+                                  double2* out, int L, double c, double a, double alpha,
+                                  double beta, hipStream_t st);
 
 }  // namespace laph

@@ -85,6 +85,7 @@ SIGNATURES = {
     "laph_shell_mult": ([_VP, _dp, _dp], _I),
     "laph_shell_get_diagonal": ([_VP, _dp], _I),
     "laph_shell_mult_device": ([_VP, _VP, _VP], _I),
+    "laph_shell_mult_noalloc": ([ctypes.POINTER(laph_params), _VP, _VP, _VP, _VP], _I),
     "laph_laplacian": ([_VP, _dp, _dp], _I),
     "laph_eps_solve": ([_VP, ctypes.POINTER(laph_eps_opts), _dp, _dp, ctypes.POINTER(laph_eps_stats)], _I),
     "laph_recover_spectrum": ([_I, _dp, _I, _D, _D, _dp], _I),
