@@ -196,6 +196,29 @@ struct View {
   int m() const { return ln->m > 0 ? ln->m : b->deg; }  // filter degree of the current cycle
 };
 
+// LAPH_TRACE: smallest squared diagonal entry of the Cholesky factors
+// (upper triangles in place, lda K) of the lane's timeslices -- with unit
+// Gram diagonals a bound above each Gram's smallest eigenvalue -- and its
+// timeslice, into ln->min_pivot2 / min_pivot_ts.
+static int trace_pivots(const View& v, const double2* F, int a) {
+  const int K = v.b->K, nt = v.ln->nt;
+  std::vector<double2> d((size_t)a * nt);
+  for (int t = 0; t < nt; ++t)
+    LAPH_HIP(hipMemcpy2DAsync(d.data() + (size_t)t * a, sizeof(double2), F + (size_t)t * K * K,
+                              (size_t)(K + 1) * sizeof(double2), sizeof(double2), a, hipMemcpyDeviceToHost,
+                              v.ln->st));
+  LAPH_HIP(hipStreamSynchronize(v.ln->st));
+  double mp = INFINITY;
+  int arg = 0;
+  for (size_t i = 0; i < d.size(); ++i)
+    if (d[i].x * d[i].x < mp) mp = d[i].x * d[i].x, arg = (int)(i / a);
+  if (mp < v.ln->min_pivot2) {  // smallest over the factorizations of the cycle
+    v.ln->min_pivot2 = mp;
+    v.ln->min_pivot_ts = v.ln->t0 + arg;
+  }
+  return LAPH_OK;
+}
+
 static int dense_sync_info(const View& v, const char* what) {
   std::vector<rocblas_int> info(v.ln->nt);
   LAPH_HIP(hipMemcpyAsync(info.data(), v.b->info + v.ln->t0, v.ln->nt * sizeof(rocblas_int), hipMemcpyDeviceToHost,
@@ -293,6 +316,7 @@ static int cholqr(const View& v, Act q, bool shifted = false) {
     LAPH_TRY(cholqr(v, q, true));
     return cholqr(v, q);
   }
+  if (v.b->trace && !shifted) LAPH_TRY(trace_pivots(v, G, q.a));  // the unit-column block's Gram
   LAPH_RB(rocblas_ztrsm_strided_batched(v.ln->rb, rocblas_side_left, rocblas_fill_upper,
                                         rocblas_operation_conjugate_transpose, rocblas_diagonal_non_unit, q.a, v.b->n,
                                         &one, zc(G), K, (rocblas_stride)K * K, zc(Y + q.l), K,
@@ -376,6 +400,7 @@ static int rayleigh_ritz(const View& v, Act q, bool orthonormal, bool* chol_fail
       *chol_failed = true;
       return LAPH_OK;
     }
+    if (b->trace) LAPH_TRY(trace_pivots(v, G2, q.a));
     if (b->use_trsm) {
       LAPH_RB(rocblas_ztrsm_strided_batched(rb, rocblas_side_left, rocblas_fill_upper,
                                             rocblas_operation_conjugate_transpose, rocblas_diagonal_non_unit, q.a,
@@ -1024,6 +1049,7 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
       }
     }
     ln->flaunch += (m - 1) + (long long)(passes - 1) * m;
+    ln->min_pivot2 = INFINITY;
     void* w = nullptr;
     LAPH_HIP(hipEventRecord(ln->ev[0], ln->st));
     LAPH_TRY(run_filter(v, passes, &w));
@@ -1052,8 +1078,10 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
       const double* rs = &b->h_res[0];
       std::fprintf(stderr,
                    "[laph] cycle %d: degree %d passes %d  window %.4f  mu[nev-1] %.5f  mu[K-1] %.4f  res[0] %.2e  "
-                   "res[nev-1] %.2e  worst %.2e  locked %d  lock %d\n", it, m, passes, lc[0], mu[b->nev - 1], mu[K - 1],
-                   std::sqrt(rs[0]), std::sqrt(rs[b->nev - 1]), worst, b->nlock[0], ln->lock);
+                   "res[nev-1] %.2e  worst %.2e  locked %d  lock %d  min pivot^2 %.2e (timeslice %d)  rr fallbacks %d  "
+                   "shifted %d\n", it, m, passes, lc[0], mu[b->nev - 1], mu[K - 1], std::sqrt(rs[0]),
+                   std::sqrt(rs[b->nev - 1]), worst, b->nlock[0], ln->lock, ln->min_pivot2, ln->min_pivot_ts,
+                   ln->rr_fallbacks, ln->shifted);
     }
     float a = 0, c = 0;
     LAPH_HIP(hipEventElapsedTime(&a, ln->ev[0], ln->ev[1]));

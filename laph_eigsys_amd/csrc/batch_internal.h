@@ -42,6 +42,9 @@ struct laph_lane {
   int recoveries = 0;  // timeslice blocks re-seeded after a failed orthonormalisation
   int rr_fallbacks = 0;  // Rayleigh-Ritz Gram Cholesky failures (block then orthonormalised by CholQR2)
   int shifted = 0;       // shifted Cholesky QR passes (plain CholQR Gram not numerically definite)
+  double min_pivot2 = 0;  // LAPH_TRACE: smallest squared Cholesky pivot of the unit-diagonal Grams
+                          // factored in the current cycle (CholQR and Rayleigh-Ritz), over the lane
+  int min_pivot_ts = -1;  // the timeslice it belongs to
   double fbytes = 0;
   float ms_filter = 0, ms_dense = 0;
 };
