@@ -168,6 +168,7 @@ def main():
     ev_batch = [b.get(t, want_vectors=False)[0] for t in range(nts)]
     # dominant kernel: FP32 filter stencil, timed alone on the device
     ms_launch, bytes_launch = b.time_filter(reps=args.filter_reps)
+    gram = gram_rate(b)
     b.close()
     achieved = bytes_launch / (ms_launch * 1e-3)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
@@ -220,6 +221,7 @@ def main():
                                f"residual <= {TOL}, ending with phase-fixed eigenpairs in HBM",
                    "lattice": f"{L}^3x{nts}", "nev": NEV, "deg": DEG, "tol": TOL, "parallelism": "none (1 GPU)"},
         "roofline": roofline,
+        "gram": gram,
         "plain_fp64": {"value": plain_value, "unit": "timeslices/s", "timeslices": len(plain_times),
                        "lambda_c": lamC, "cycles_max": max(plain_cycles) if plain_cycles else None,
                        "seconds_per_timeslice": spread(plain_times),
@@ -464,6 +466,7 @@ def workload_c4(lp, args):
         print(f"[bench] c4 group {g0}..{g0 + n - 1}: {st.seconds:.2f}s cycles={st.iterations}", file=sys.stderr,
               flush=True)
     ms, by = b.time_filter(reps=5)
+    gram = gram_rate(b)
     # streaming device -> file of two timeslices' eigensystems (1.06 GB of
     # eigenvectors each) into a scratch directory that is removed afterwards
     import shutil
@@ -483,7 +486,7 @@ def workload_c4(lp, args):
                       "groups": [n for _, n in groups], "ts_per_s_solve": T4 / t_solve,
                       "ts_per_s_with_upload_hyp": T4 / (t_solve + t_set), "cycles_max": cycles,
                       "max_residual": maxres, "filter_block": st.filter_block, "filter_strip": st.filter_strip,
-                      "filter_GBps": by / ms / 1e6, "filter_us_per_launch": ms * 1e3}))
+                      "filter_GBps": by / ms / 1e6, "filter_us_per_launch": ms * 1e3, "gram": gram}))
 
 
 def stream_ceiling():
@@ -509,6 +512,15 @@ def device_sync():
         if rc != 0:
             raise RuntimeError(f"hipDeviceSynchronize failed with {rc}")
     return sync
+
+
+def gram_rate(b, reps=10):
+    """The FP64 matrix-core kernel of the dense phase: the Gram product of the
+    first lane's timeslices at full width, timed alone on the device."""
+    ms, flops = b.time_gram(reps=reps)
+    rate = flops / (ms * 1e-3)
+    return {"kernel": "strided-batched zgemm, Hermitian upper-triangle row blocks", "ms": ms,
+            "tflop_executed": flops / 1e12, "TFLOPps": rate / 1e12, "frac_fp64_mfma_peak": rate / PEAK_FP64_MFMA}
 
 
 def plain_lambda_c(L, nev, factor=1.35):

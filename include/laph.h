@@ -212,6 +212,8 @@ typedef struct laph_batch laph_batch;
  *   LAPH_TRSM=1             Rayleigh-Ritz through triangular solves with the Gram's Cholesky factor
  *                           (else its explicit inverse and GEMMs)
  *   LAPH_EIG=0|1|2          Rayleigh-Ritz eigensolver: zheevd, zheevdj, zheevj (0)
+ *   LAPH_GRAM_SPLIT=s       row blocks of the Hermitian Gram and Rayleigh-quotient products, upper
+ *                           triangle only (2); 1: the full products
  *   LAPH_TRACE=1            per-cycle trace of the first timeslice on stderr
  *   LAPH_TEST_POISON=t      (tests) NaN written into one row of timeslice t's unlocked columns in
  *                           cycle 2, to exercise the recovery counted in laph_batch_stats */
@@ -301,6 +303,13 @@ int laph_source_shape(int L, int nev, const double* V, int device, double* psi);
  * or written once per step, links once per 64-vector block).  Requires a
  * solved batch (its coefficient tables).  No counterpart in the reference. */
 int laph_batch_time_filter(laph_batch* b, int reps, double* ms_per_launch, double* bytes_per_launch);
+/* Device time of the Gram product Y^H Y of the first lane's timeslices at
+ * full width (the solve's FP64 matrix-core kernel, strided-batched zgemm of
+ * the upper-triangle row blocks), alone on the device: reps replays timed
+ * with HIP events on the lane's stream.  Outputs the average duration and
+ * the floating-point operations it executes.  Requires a solved batch (the
+ * block it multiplies).  No counterpart in the reference. */
+int laph_batch_time_gram(laph_batch* b, int reps, double* ms_per_gram, double* flops_per_gram);
 /* Results of timeslice ts: eigenvalues of D (descending, i.e. nearest 0
  * first, like recover_spectrum's output), phase-fixed eigenvectors (n x nev
  * complex column-major, may be NULL), phases (may be NULL), per-pair

@@ -107,6 +107,7 @@ SIGNATURES = {
     "laph_batch_solve": ([_VP, ctypes.POINTER(laph_batch_stats)], _I),
     "laph_batch_get": ([_VP, _I, _dp, _dp, _dp, _dp], _I),
     "laph_batch_time_filter": ([_VP, _I, _dp, _dp], _I),
+    "laph_batch_time_gram": ([_VP, _I, _dp, _dp], _I),
     "laph_batch_write": ([_VP, _I, _S, _I, _I, _dp], _I),
     "laph_batch_source_shape": ([_VP, _I, _dp], _I),
     "laph_source_shape": ([_I, _I, _dp, _I, _dp], _I),
@@ -391,6 +392,12 @@ class Batch:
         ms, by = ctypes.c_double(), ctypes.c_double()
         _check(lib().laph_batch_time_filter(self.h, reps, ctypes.byref(ms), ctypes.byref(by)), "laph_batch_time_filter")
         return ms.value, by.value
+
+    def time_gram(self, reps=10):
+        """(ms per Gram, flops it executes): the FP64 matrix-core product alone on the device."""
+        ms, fl = ctypes.c_double(), ctypes.c_double()
+        _check(lib().laph_batch_time_gram(self.h, reps, ctypes.byref(ms), ctypes.byref(fl)), "laph_batch_time_gram")
+        return ms.value, fl.value
 
     def write(self, ts, result_path, config, t):
         """eigenvectors / eigenvalues / phases files of timeslice ts (file index t)
