@@ -215,6 +215,8 @@ typedef struct laph_batch laph_batch;
  *   LAPH_GRAM_SPLIT=s       row blocks of the Hermitian Gram and Rayleigh-quotient products, upper
  *                           triangle only (2); 1: the full products
  *   LAPH_TRACE=1            per-cycle trace of the first timeslice on stderr
+ *   LAPH_NET_TARGET=1       laph_net_train_step fits the span of all K Ritz vectors of the last solve
+ *                           (else its nev eigenvectors)
  *   LAPH_TEST_POISON=t      (tests) NaN written into one row of timeslice t's unlocked columns in
  *                           cycle 2, to exercise the recovery counted in laph_batch_stats */
 
@@ -225,7 +227,9 @@ typedef struct laph_batch_opts {
   double tol;        /* lock when |D v - lambda v| <= tol (v unit norm) */
   int max_iter;      /* filter / Rayleigh-Ritz cycles (0 = 60) */
   unsigned long long seed;
-  int warm_start;    /* 1: start from the block proposed by laph_net_propose, when one was */
+  int warm_start;    /* 1: start from the block proposed by laph_net_propose, when one was (Rayleigh-Ritz
+                        on it, then the usual warm-up cycle); 2: the same without the warm-up cycle, the
+                        proposal's Ritz values setting the first window */
   int filter_passes; /* 0: after one warm-up cycle of T_DEG(B), one Chebyshev filter per cycle of
                         the highest degree (a multiple of 4, at most 72) whose amplification
                         T_d(theta_max) stays below 1e10; 1 .. 6: up to that many applications of

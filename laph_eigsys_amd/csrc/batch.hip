@@ -1060,7 +1060,9 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
   for (it = 0; it < max_iter && active > 0; ++it) {
     // one filter pass while the window settles, then T_m(B)^P per cycle; in
     // the per-cycle degree mode one pass of the degree the window allows
-    const bool warmup = it < b->warm_cycles;
+    // a network proposal (warm_start = 2) replaces the warm-up cycles: its
+    // Rayleigh-Ritz values give the window of the first cycle
+    const bool warmup = it < (warm && b->o.warm_start == 2 ? 0 : b->warm_cycles);
     const bool wdeg = b->fdeg_max >= b->warm_deg && b->warm_deg > 0;  // warm-up degree of the per-cycle mode
     ln->m = b->fdeg_max > 0 && !warmup ? cycle_degree(b, lc) : (warmup && wdeg ? b->warm_deg : b->deg);
     const int passes = b->fdeg_max > 0 || warmup ? 1 : filter_passes(b, lc);
@@ -1290,6 +1292,8 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
   LAPH_CHECK(o->tol > 0, LAPH_ERR_INVALID, "laph_batch_create: tol must be positive");
   LAPH_CHECK(o->filter_passes >= 0 && o->filter_passes <= LAPH_MAX_PASSES, LAPH_ERR_INVALID,
              "laph_batch_create: filter_passes=%d outside 0..%d", o->filter_passes, LAPH_MAX_PASSES);
+  LAPH_CHECK(o->warm_start >= 0 && o->warm_start <= 2, LAPH_ERR_INVALID, "laph_batch_create: warm_start=%d outside 0..2",
+             o->warm_start);
   LAPH_CHECK(o->filter_storage == 0 || o->filter_storage == 1, LAPH_ERR_INVALID,
              "laph_batch_create: filter_storage=%d (0 FP32, 1 BF16)", o->filter_storage);
   const int n = 3 * p->L * p->L * p->L;
@@ -1365,6 +1369,7 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     b->esz = b->bf16 ? 4 : 8;
     if (const char* e = std::getenv("LAPH_EIG")) b->eig_mode = std::atoi(e);
     if (const char* e = std::getenv("LAPH_GRAM_SPLIT")) b->gram_split = std::max(1, std::min(8, std::atoi(e)));
+    if (const char* e = std::getenv("LAPH_NET_TARGET")) b->net_target_all = std::atoi(e) != 0;
     b->nlock.assign(b->nts, 0);
     b->h_mu.assign((size_t)b->nts * K, 0.0);
     b->h_res.assign((size_t)b->nts * K, 0.0);

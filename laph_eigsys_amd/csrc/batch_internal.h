@@ -127,6 +127,7 @@ struct laph_batch {
   bool use_trsm = false;     // triangular solves instead of the explicit inverse in Rayleigh-Ritz (LAPH_TRSM=1)
   bool start_rr = false;     // Rayleigh-Ritz on the Gaussian start block before the first cycle (LAPH_START_RR=1)
   bool force_shift = false;  // every CholQR pass starts with a shifted one (LAPH_SHIFTED_CHOLQR)
+  bool net_target_all = false;  // laph_net_train_step targets all K converged Ritz vectors (LAPH_NET_TARGET=1)
   int gram_split = 2;        // row blocks of the Hermitian Gram / Rayleigh-quotient products (LAPH_GRAM_SPLIT;
                              // 1: the full product)
 };

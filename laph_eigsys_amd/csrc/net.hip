@@ -780,12 +780,13 @@ int laph_net_train_step(laph_net* nt, laph_batch* b, unsigned long long seed, do
     LAPH_HIP(hipStreamSynchronize(b->st));
     const int K = b->K, L = b->p.L, nts = b->nts;
     const size_t tot = b->per_ts * nts;
+    const int tgt = b->net_target_all ? K : b->nev;  // target span: the nev eigenvectors or all K Ritz vectors
     laph::NetBufs<laph::bf16c> B;
-    LAPH_TRY(laph::net_bufs<laph::bf16c>(nt, L, K, nts, b->nev, true, B));
+    LAPH_TRY(laph::net_bufs<laph::bf16c>(nt, L, K, nts, tgt, true, B));
     laph::k_net_input<laph::bf16c><<<(unsigned)((tot + 255) / 256), 256, 0, nt->st>>>(B.act[0], tot, seed, 0);
     LAPH_HIP(hipGetLastError());
     LAPH_TRY(laph::net_forward<laph::bf16c>(nt, b->U32, L, K, nts, B, true));
-    LAPH_TRY(laph::net_loss<laph::bf16c>(nt, b->n, K, nts, b->B[0], K, b->per_ts, b->nev, B, loss));
+    LAPH_TRY(laph::net_loss<laph::bf16c>(nt, b->n, K, nts, b->B[0], K, b->per_ts, tgt, B, loss));
     std::vector<float> grad;
     LAPH_TRY(laph::net_backward<laph::bf16c>(nt, b->U32, L, K, nts, B, grad));
     laph::adam(nt, grad, lr);
