@@ -212,8 +212,9 @@ typedef struct laph_batch laph_batch;
  *   LAPH_TRSM=1             Rayleigh-Ritz through triangular solves with the Gram's Cholesky factor
  *                           (else its explicit inverse and GEMMs)
  *   LAPH_EIG=0|1|2          Rayleigh-Ritz eigensolver: zheevd, zheevdj, zheevj (0)
- *   LAPH_GRAM_SPLIT=s       row blocks of the Hermitian Gram and Rayleigh-quotient products, upper
- *                           triangle only (2); 1: the full products
+ *   LAPH_GRAM_SPLIT=s[,c]   row blocks s of the Hermitian Gram and Rayleigh-quotient products (upper
+ *                           triangle only; 1: the full products) and site chunks c (1, 2, 4) of their
+ *                           sums (else both timed on the first solve)
  *   LAPH_TRACE=1            per-cycle trace of the first timeslice on stderr
  *   LAPH_NET_TARGET=1       laph_net_train_step fits the span of all K Ritz vectors of the last solve
  *                           (else its nev eigenvectors)
@@ -269,6 +270,10 @@ typedef struct laph_batch_stats {
                               block was orthonormalised by explicit CholQR2 */
   int shifted_cholqr;      /* shifted Cholesky QR passes (a CholQR Gram that was not numerically
                               positive definite) */
+  int gram_blocks;         /* row blocks of the Hermitian Gram products (1: full zgemm, 2: upper
+                              triangle in two row blocks), timed on the first solve */
+  int gram_chunks;         /* site chunks of the Gram products (1, 2 or 4: partial products as
+                              extra batch entries, added in order), timed with gram_blocks */
 } laph_batch_stats;
 
 int laph_batch_create(laph_batch** out, const laph_params* params, const laph_batch_opts* opts);
@@ -307,6 +312,14 @@ int laph_source_shape(int L, int nev, const double* V, int device, double* psi);
  * or written once per step, links once per 64-vector block).  Requires a
  * solved batch (its coefficient tables).  No counterpart in the reference. */
 int laph_batch_time_filter(laph_batch* b, int reps, double* ms_per_launch, double* bytes_per_launch);
+/* Device-pointer variant of laph_batch_get for the eigenvectors: the nev
+ * phase-fixed eigenvectors of timeslice ts in the eigenvector-file layout (n x
+ * nev complex column-major) into d_V, device memory of n*nev complex doubles
+ * on the batch's device (caller-owned), and the phases into the host array
+ * phase (may be NULL).  For keeping the eigensystems of every timeslice of a
+ * configuration resident while the solver's work buffers serve groups of
+ * timeslices in turn.  Completes before returning. */
+int laph_batch_get_device(laph_batch* b, int ts, double* d_V, double* phase);
 /* Device time of the Gram product Y^H Y of the first lane's timeslices at
  * full width (the solve's FP64 matrix-core kernel, strided-batched zgemm of
  * the upper-triangle row blocks), alone on the device: reps replays timed

@@ -58,11 +58,11 @@
 
 namespace laph {
 
-// Candidate block widths of the filter buffers: {64, 128} for FP32
-// storage, {128, 256} for BF16 (the narrower one divides every K).
-static void block_widths(const laph_batch* b, int cand[2]) {
-  cand[0] = b->bf16 ? 128 : 64;
-  cand[1] = b->bf16 ? 256 : 128;
+// Candidate block widths of the filter buffers: {32, 64, 128} for FP32
+// storage, {128, 256} for BF16 (the narrowest divides every K).
+static std::vector<int> block_widths(const laph_batch* b) {
+  if (b->bf16) return {128, 256};
+  return {32, 64, 128};
 }
 
 // Filter buffer `base` at entry offset `off` (entries of esz bytes).
@@ -83,6 +83,7 @@ static int batch_alloc(laph_batch* b) {
   LAPH_HIP(hipMalloc(&b->G2, kk * sizeof(double2)));
   LAPH_HIP(hipMalloc(&b->P2, kk * sizeof(double2)));
   LAPH_HIP(hipMalloc(&b->G3, kk * sizeof(double2)));
+  LAPH_HIP(hipMalloc(&b->Gpart, kk * b->gram_ksplit_max * sizeof(double2)));
   LAPH_HIP(hipMemset(b->G3, 0, kk * sizeof(double2)));
   const size_t nk = (size_t)b->nts * b->K;
   LAPH_HIP(hipMalloc(&b->mu, nk * sizeof(double)));
@@ -107,15 +108,12 @@ static int batch_alloc(laph_batch* b) {
   LAPH_HIP(hipMemset(b->first_blk, 0, b->nts * sizeof(int)));
   LAPH_HIP(hipMemset(b->R32, 0, nb * b->esz));
   for (int i = 0; i < 3; ++i) LAPH_HIP(hipMemset(b->P[i], 0, nb * b->esz));
-  int cand[2];
-  block_widths(b, cand);
+  const std::vector<int> cand = block_widths(b);
   if (const char* e = std::getenv("LAPH_BW")) {
     const int w = std::atoi(e);
-    LAPH_CHECK((w == cand[0] || w == cand[1]) && b->K % w == 0, LAPH_ERR_INVALID,
-               "LAPH_BW=%s: %d or %d dividing K=%d", e, cand[0], cand[1], b->K);
+    LAPH_CHECK(std::find(cand.begin(), cand.end(), w) != cand.end() && b->K % w == 0, LAPH_ERR_INVALID,
+               "LAPH_BW=%s: one of %s dividing K=%d", e, b->bf16 ? "128, 256" : "32, 64, 128", b->K);
     b->bw = w;
-  } else if (b->K % cand[1] != 0) {
-    b->bw = cand[0];
   }
   int nlanes = 1;
   if (const char* e = std::getenv("LAPH_LANES")) nlanes = std::max(1, std::min(4, std::atoi(e)));
@@ -160,7 +158,7 @@ static void batch_free(laph_batch* b) {
   void* bufs[] = {b->U64, b->U32, b->dec,  b->Unew, b->B[0], b->B[1],   b->B[2], b->R32,     b->P[0],
                   b->P[1], b->P[2], b->G,  b->G2,   b->P2,   b->G3,   b->mu,     b->E,    b->info,    b->nrm2,
                   b->gam, b->coef, b->cab, b->first_blk, b->lock_blk, b->cn, b->coef2, b->jsweeps,
-                  b->red, b->phase};
+                  b->red, b->phase, b->Gpart};
   for (void* q : bufs)
     if (q) (void)hipFree(q);
   if (b->st) (void)hipStreamDestroy(b->st);
@@ -245,16 +243,47 @@ static int gram_rows(int a, int split) {
 // 47.0 ms per Gram of 64 timeslices at 32^3; 4 row blocks of 64 rows fall to
 // 56% of the FP64 matrix peak).  gram_split = 1 computes the full product.
 // Entries strictly below the diagonal blocks are left as they were.
+// With gram_ksplit = S > 1 the site sum is cut into S chunks that enter the
+// same strided-batched call as extra batch entries (timeslice t, chunk c ->
+// entry t*S + c: the chunks of n/S rows are (n/S)*K apart, the timeslices
+// n*K = S*(n/S)*K apart, one stride for both), each writing its partial
+// product into Gpart; k_sum_gram_chunks adds the S partials in chunk order.
+// That multiplies the output tiles of a call by S when a lane holds few
+// timeslices (24 per group at 48^3).
+__global__ void k_sum_gram_chunks(const double2* __restrict__ P, double2* __restrict__ G, int a, int K, int S,
+                                  int bs) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a * a) return;
+  const int i = idx % a, j = idx / a;
+  if (j < (i / bs) * bs) return;  // below the computed row blocks
+  const double2* p = P + (size_t)blockIdx.y * S * K * K + (size_t)j * K + i;
+  double2 acc = make_double2(0.0, 0.0);
+  for (int c = 0; c < S; ++c) {
+    const double2 u = p[(size_t)c * K * K];
+    acc.x += u.x;
+    acc.y += u.y;
+  }
+  G[(size_t)blockIdx.y * K * K + (size_t)j * K + i] = acc;
+}
+
 static int gram(const View& v, Act q, const double2* A, const double2* Bm, double2* Gt, double alpha) {
   const int K = v.b->K;
   const cplx al(alpha, 0.0), zero(0.0, 0.0);
   const int bs = gram_rows(q.a, std::max(1, v.b->gram_split));
+  const int S = v.b->gram_ksplit;
+  const int rows = v.b->n / S;  // S divides n
+  double2* C = S > 1 ? v.b->Gpart + (size_t)v.ln->t0 * S * K * K : Gt;
   for (int r0 = 0; r0 < q.a; r0 += bs) {
     const int m = std::min(bs, q.a - r0);
     LAPH_RB(rocblas_zgemm_strided_batched(v.ln->rb, rocblas_operation_none, rocblas_operation_conjugate_transpose, m,
-                                          q.a - r0, v.b->n, &al, zc(A + q.l + r0), K, (rocblas_stride)v.b->per_ts,
-                                          zc(Bm + q.l + r0), K, (rocblas_stride)v.b->per_ts, &zero,
-                                          zc(Gt + r0 + (size_t)r0 * K), K, (rocblas_stride)K * K, v.ln->nt));
+                                          q.a - r0, rows, &al, zc(A + q.l + r0), K, (rocblas_stride)rows * K,
+                                          zc(Bm + q.l + r0), K, (rocblas_stride)rows * K, &zero,
+                                          zc(C + r0 + (size_t)r0 * K), K, (rocblas_stride)K * K, v.ln->nt * S));
+  }
+  if (S > 1) {
+    k_sum_gram_chunks<<<dim3((unsigned)((q.a * q.a + 255) / 256), v.ln->nt), 256, 0, v.ln->st>>>(C, Gt, q.a, K, S,
+                                                                                                   bs);
+    LAPH_HIP(hipGetLastError());
   }
   return LAPH_OK;
 }
@@ -1170,10 +1199,8 @@ static int autotune_block_width(laph_batch* b) {
   laph_lane* ln = &b->lanes[0];
   ln->m = 0;  // layouts are timed at the batch's degree
   LAPH_HIP(hipMemsetAsync(b->first_blk + ln->t0, 0, ln->nt * sizeof(int), ln->st));
-  int cand[2];
-  block_widths(b, cand);
   std::vector<int> widths, strips;
-  for (int w : cand)
+  for (int w : block_widths(b))
     if (b->K % w == 0 && (b->bw == 0 || b->bw == w)) widths.push_back(w);
   if (b->strip_fixed) {
     strips.push_back(b->strip);
@@ -1201,9 +1228,39 @@ static int autotune_block_width(laph_batch* b) {
   b->bw = wbest;
   b->strip = sbest;
   b->tuned = true;
+  if (!b->gram_split_fixed) {
+    // Gram row blocks: the upper-triangle split saves a quarter of the flops
+    // but launches fewer tiles per GEMM, which leaves CUs idle when the lane
+    // holds few timeslices (24 at 48^3: 55% of the FP64 matrix peak against
+    // 90% with 64 timeslices at 32^3); timed on the lane's block at full width
+    const View v{b, ln};
+    const Act q{0, b->K};
+    float best_ms = 0;
+    int best_split = 1, best_ks = 1;
+    bool first_g = true;
+    for (int split : {1, 2})
+      for (int ks : {1, 2, 4}) {
+        if (b->n % ks != 0 || ks > b->gram_ksplit_max) continue;
+        b->gram_split = split;
+        b->gram_ksplit = ks;
+        LAPH_TRY(gram(v, q, v.X(), v.X(), b->G2 + v.kk(), 1.0));  // warm-up
+        LAPH_HIP(hipEventRecord(ln->ev[0], ln->st));
+        for (int r = 0; r < 2; ++r) LAPH_TRY(gram(v, q, v.X(), v.X(), b->G2 + v.kk(), 1.0));
+        LAPH_HIP(hipEventRecord(ln->ev[1], ln->st));
+        LAPH_HIP(hipEventSynchronize(ln->ev[1]));
+        float ms = 0;
+        LAPH_HIP(hipEventElapsedTime(&ms, ln->ev[0], ln->ev[1]));
+        if (first_g || ms < best_ms) best_ms = ms, best_split = split, best_ks = ks;
+        first_g = false;
+      }
+    b->gram_split = best_split;
+    b->gram_ksplit = best_ks;
+  }
   LAPH_HIP(hipGraphExecDestroy(ln->graph));  // captured for the last layout timed
   ln->graph = nullptr;
-  if (b->trace) std::fprintf(stderr, "[laph] filter block width %d, strip %d\n", wbest, sbest);
+  if (b->trace)
+    std::fprintf(stderr, "[laph] filter block width %d, strip %d, Gram row blocks %d, site chunks %d\n", wbest, sbest,
+                 b->gram_split, b->gram_ksplit);
   return LAPH_OK;
 }
 
@@ -1265,6 +1322,8 @@ static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
   }
   s.iterations = cycles;
   s.filter_block = b->bw;
+  s.gram_blocks = b->gram_split;
+  s.gram_chunks = b->gram_ksplit;
   s.filter_strip = b->strip;
   s.filter_zrun = b->bf16 == 0 && b->bw == 128 ? laph::stencil_zrun(b) : 0;
   s.start_window = b->window0;
@@ -1368,7 +1427,13 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     b->bf16 = o->filter_storage == 1;
     b->esz = b->bf16 ? 4 : 8;
     if (const char* e = std::getenv("LAPH_EIG")) b->eig_mode = std::atoi(e);
-    if (const char* e = std::getenv("LAPH_GRAM_SPLIT")) b->gram_split = std::max(1, std::min(8, std::atoi(e)));
+    if (const char* e = std::getenv("LAPH_GRAM_SPLIT")) {  // "rows" or "rows,chunks"
+      b->gram_split = std::max(1, std::min(8, std::atoi(e)));
+      const char* c = std::strchr(e, ',');
+      b->gram_ksplit = c ? std::max(1, std::min(b->gram_ksplit_max, std::atoi(c + 1))) : 1;
+      if (b->n % b->gram_ksplit != 0) b->gram_ksplit = 1;
+      b->gram_split_fixed = true;
+    }
     if (const char* e = std::getenv("LAPH_NET_TARGET")) b->net_target_all = std::atoi(e) != 0;
     b->nlock.assign(b->nts, 0);
     b->h_mu.assign((size_t)b->nts * K, 0.0);
@@ -1510,6 +1575,18 @@ int laph_batch_get(laph_batch* b, int ts, double* evals, double* V_fix, double* 
     (void)hipFree(dV);
     LAPH_HIP(e);
   }
+  return LAPH_OK;
+}
+
+int laph_batch_get_device(laph_batch* b, int ts, double* d_V, double* phase) {
+  LAPH_CHECK(b && ts >= 0 && ts < b->nts && d_V, LAPH_ERR_INVALID, "laph_batch_get_device: bad arguments");
+  LAPH_CHECK(b->solved, LAPH_ERR_INVALID, "laph_batch_get_device: batch not solved");
+  LAPH_HIP(hipSetDevice(b->p.device));
+  LAPH_HIP(laph::launch_extract_cols(b->B[0] + (size_t)ts * b->per_ts, reinterpret_cast<double2*>(d_V), b->n, b->K,
+                                     b->nev, b->st));
+  LAPH_HIP(hipStreamSynchronize(b->st));
+  if (phase)
+    for (int j = 0; j < b->nev; ++j) phase[j] = b->h_phase[(size_t)ts * b->nev + j];
   return LAPH_OK;
 }
 

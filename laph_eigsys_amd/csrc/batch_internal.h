@@ -128,8 +128,12 @@ struct laph_batch {
   bool start_rr = false;     // Rayleigh-Ritz on the Gaussian start block before the first cycle (LAPH_START_RR=1)
   bool force_shift = false;  // every CholQR pass starts with a shifted one (LAPH_SHIFTED_CHOLQR)
   bool net_target_all = false;  // laph_net_train_step targets all K converged Ritz vectors (LAPH_NET_TARGET=1)
-  int gram_split = 2;        // row blocks of the Hermitian Gram / Rayleigh-quotient products (LAPH_GRAM_SPLIT;
-                             // 1: the full product)
+  int gram_split = 2;        // row blocks of the Hermitian Gram / Rayleigh-quotient products, 1 (the full
+                             // product) or 2, timed on the first solve (LAPH_GRAM_SPLIT fixes it)
+  bool gram_split_fixed = false;
+  int gram_ksplit = 1;       // site chunks of the Gram products (1, 2 or 4), timed with gram_split
+  int gram_ksplit_max = 4;
+  double2* Gpart = nullptr;  // [nts][gram_ksplit_max][K*K] partial Gram products of the site chunks
 };
 
 namespace laph {

@@ -201,6 +201,59 @@ __global__ __launch_bounds__(256) void k_filter_step32(const float2* __restrict_
   }
 }
 
+// 32-vector-block variant (filter buffers block-major with 32-vector blocks,
+// 256 B per site row): each wavefront holds two sites, one per half, so a
+// work-group covers 8 consecutive sites of the sweep.  The two halves read
+// different sites' links, through vector loads instead of wave-uniform scalar
+// loads; the links are read once per 32 vectors.
+template <bool HAS_R>
+__global__ __launch_bounds__(256) void k_filter_step32h(const float2* __restrict__ U32,
+                                                         const float2* __restrict__ in,
+                                                         const float2* __restrict__ r,
+                                                         const float2* __restrict__ old,
+                                                         float2* __restrict__ out,
+                                                         const float4* __restrict__ coef,
+                                                         const float2* __restrict__ cab,
+                                                         const int* __restrict__ first_blk, int L,
+                                                         int K, int nts, int use_old, int W) {
+  const int V3 = L * L * L;
+  int ts, cb, sb;
+  unit_map((V3 + 7) / 8, K / 32, nts, ts, cb, sb);
+  if (cb * 32 + 31 < first_blk[ts] * 64) return;  // the whole 32-vector block is locked
+  const int l32 = threadIdx.x & 31;
+  const int si = sb * 8 + (threadIdx.x >> 5);
+  if (si >= V3) return;
+  const int s = strip_site(si, L, W);
+  const int col = cb * 32 + l32;
+  const size_t vo = ((size_t)ts * (K / 32) + cb) * (size_t)V3 * 3 * 32;
+  const float2* Ut = U32 + (size_t)ts * V3 * 27;
+  float2 acc[3], ctr[3];
+  hop<float2>(Ut, in + vo, s, L, 32, l32, acc, ctr);
+  const float4 cf = coef[(size_t)ts * K + col];
+  const float2 ca = cab[ts];  // (c_t, a_t - 6 c_t)
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t idx = vo + (size_t)(3 * s + c) * 32 + l32;
+    float2 o;
+    o.x = cf.x * (ca.x * acc[c].x + ca.y * ctr[c].x) + cf.w * ctr[c].x;
+    o.y = cf.x * (ca.x * acc[c].y + ca.y * ctr[c].y) + cf.w * ctr[c].y;
+    if (HAS_R) {
+      const float2 rv = ld_nt(r + idx);
+      o.x += cf.y * rv.x;
+      o.y += cf.y * rv.y;
+    }
+    if (use_old) {
+      const float2 ov = ld_nt(old + idx);
+      o.x += cf.z * ov.x;
+      o.y += cf.z * ov.y;
+    } else {  // imaginary part of a complex shift
+      o.x -= cf.z * ctr[c].y;
+      o.y += cf.z * ctr[c].x;
+    }
+    st_nt(out + idx, o);
+  }
+}
+
 // 128-vector-block variant: each lane carries two neighbouring vectors as one
 // 16-byte float4 (buffers block-major with 128-vector blocks), halving the
 // wavefronts and the load instructions per vector.
@@ -792,14 +845,17 @@ __global__ __launch_bounds__(256) void k_lap64y(const double2* __restrict__ U64,
           st_nt(out + (size_t)ts * V3 * 3 * K + cb * 64 + (unsigned)((3 * su + c) * K) + l64, o);
         }
       } else {
-        // block-major filter storage: block cb*64/BW, offset (cb*64) % BW + l64 within its rows
+        // block-major filter storage: block (cb*64 + l64)/BW, offset (cb*64 + l64) % BW within its rows
+        // (a 64-column group lies in one block for BW >= 64, in two for BW = 32)
         const size_t n = (size_t)3 * V3;
-        const size_t rbase = ((size_t)ts * (K / BW) + (cb * 64) / BW) * n * BW + (cb * 64) % BW;
+        const int gc = cb * 64 + (BW < 64 ? l64 : 0);
+        const size_t rbase = ((size_t)ts * (K / BW) + gc / BW) * n * BW + gc % BW;
+        const int lo = BW < 64 ? 0 : l64;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
           const float2 rv =
               make_float2((float)(cs * (acc[a].x + mc * zc[a].x)), (float)(cs * (acc[a].y + mc * zc[a].y)));
-          const size_t ri = rbase + (unsigned)((3 * su + a) * BW) + l64;
+          const size_t ri = rbase + (unsigned)((3 * su + a) * BW) + lo;
           if (BF16)
             static_cast<unsigned*>(rbuf)[ri] = bf2_pack(rv);
           else
@@ -1007,6 +1063,17 @@ hipError_t launch_filter_step(const float2* U32, const void* in_, const void* r_
                                                   old != nullptr, W);
     return hipGetLastError();
   }
+  if (bw == 32) {
+    const dim3 g8((unsigned)((L * L * L + 7) / 8) * (unsigned)(K / 32) * (unsigned)nts);
+    if (r)
+      k_filter_step32h<true><<<g8, 256, 0, st>>>(U32, in, r, old, out, coef, cab, first_blk, L, K, nts,
+                                                  old != nullptr, W);
+    else
+      k_filter_step32h<false><<<g8, 256, 0, st>>>(U32, in, r, old, out, coef, cab, first_blk, L, K, nts,
+                                                   old != nullptr, W);
+    return hipGetLastError();
+  }
+  if (bw != 64) return hipErrorInvalidValue;
   if (r)
     k_filter_step32<true><<<site_grid(L, K, nts), 256, 0, st>>>(U32, in, r, old, out, coef, cab, first_blk, L, K, nts,
                                                               old != nullptr, W);
@@ -1065,6 +1132,8 @@ hipError_t launch_lap_residual64(const double2* U64, const double2* X, const dou
       return launch_lap64y<true, 128, false>(U64, X, nullptr, mu, lamL, r, L, K, nts, first_blk, W, zrun, st);
     if (!bf16 && bw == 64)
       return launch_lap64y<true, 64, false>(U64, X, nullptr, mu, lamL, r, L, K, nts, first_blk, W, zrun, st);
+    if (!bf16 && bw == 32)
+      return launch_lap64y<true, 32, false>(U64, X, nullptr, mu, lamL, r, L, K, nts, first_blk, W, zrun, st);
     return hipErrorInvalidValue;
   }
   k_lap_residual64<<<site_grid(L, K, nts), 256, 0, st>>>(U64, X, mu, lamL, r, L, K, nts, first_blk, W, bw, bf16);

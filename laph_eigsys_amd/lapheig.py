@@ -64,7 +64,8 @@ class laph_batch_stats(ctypes.Structure):
                 ("filter_launches", ctypes.c_longlong), ("filter_bytes", ctypes.c_double),
                 ("filter_block", ctypes.c_int), ("filter_strip", ctypes.c_int), ("recoveries", ctypes.c_int),
                 ("filter_zrun", ctypes.c_int), ("start_window", ctypes.c_double),
-                ("rr_cholqr_fallbacks", ctypes.c_int), ("shifted_cholqr", ctypes.c_int)]
+                ("rr_cholqr_fallbacks", ctypes.c_int), ("shifted_cholqr", ctypes.c_int), ("gram_blocks", ctypes.c_int),
+                ("gram_chunks", ctypes.c_int)]
 
 
 # (name, argtypes) of every entry point declared in include/laph.h
@@ -106,6 +107,7 @@ SIGNATURES = {
     "laph_batch_set_links": ([_VP, _I, _dp], _I),
     "laph_batch_solve": ([_VP, ctypes.POINTER(laph_batch_stats)], _I),
     "laph_batch_get": ([_VP, _I, _dp, _dp, _dp, _dp], _I),
+    "laph_batch_get_device": ([_VP, _I, _VP, _dp], _I),
     "laph_batch_time_filter": ([_VP, _I, _dp, _dp], _I),
     "laph_batch_time_gram": ([_VP, _I, _dp, _dp], _I),
     "laph_batch_write": ([_VP, _I, _S, _I, _I, _dp], _I),
@@ -412,6 +414,13 @@ class Batch:
         psi = np.empty(half ** 3)
         _check(lib().laph_batch_source_shape(self.h, ts, _p(psi)), "laph_batch_source_shape")
         return psi
+
+    def get_device(self, ts, d_V):
+        """Phase-fixed eigenvectors of timeslice ts (file layout, n x nev) into
+        the device buffer at address d_V; returns the phases."""
+        ph = np.empty(self.nev)
+        _check(lib().laph_batch_get_device(self.h, ts, ctypes.c_void_p(d_V), _p(ph)), "laph_batch_get_device")
+        return ph
 
     def get(self, ts, want_vectors=True):
         ev = np.empty(self.nev)

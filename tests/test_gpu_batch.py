@@ -151,12 +151,13 @@ def test_rough_field_start_window(lp, gpu, window0, monkeypatch):
     b.close()
 
 
-@pytest.mark.parametrize("storage,width", [(0, 64), (0, 128), (1, 128), (1, 256)])
+@pytest.mark.parametrize("storage,width", [(0, 32), (0, 64), (0, 128), (1, 128), (1, 256)])
 def test_filter_block_widths(lp, gpu, storage, width, monkeypatch):
-    """Every layout of the filter buffers (FP32 storage: one or two vectors
-    per lane in 64- / 128-vector blocks; BF16 storage: two or four per lane
-    in 128- / 256-vector blocks) gives the dense oracle's spectrum, and the
-    timing autotuner picks one of the two widths of the storage format."""
+    """Every layout of the filter buffers (FP32 storage: two sites per
+    wavefront in 32-vector blocks, one or two vectors per lane in 64- /
+    128-vector blocks; BF16 storage: two or four per lane in 128- /
+    256-vector blocks) gives the dense oracle's spectrum, and the timing
+    autotuner picks one of the widths of the storage format."""
     L, nts, nev, deg, tol = 8, 2, 60, 10, 1e-10
     data = smeared(L, 99, 0.35, nts)
     monkeypatch.setenv("LAPH_BW", str(width))
@@ -173,7 +174,28 @@ def test_filter_block_widths(lp, gpu, storage, width, monkeypatch):
     b = lp.Batch(L, deg, 12.0, 0.0, nts, nev, nsub=256, tol=tol, seed=5, device=gpu, filter_storage=storage)
     for t, (ts, _) in enumerate(data):
         b.set_timeslice(t, ts, 0.62, 0.62, 3)
-    assert b.solve().filter_block in ((64, 128) if storage == 0 else (128, 256))
+    assert b.solve().filter_block in ((32, 64, 128) if storage == 0 else (128, 256))
+    b.close()
+
+
+@pytest.mark.parametrize("L,zrun", [(9, 0), (12, 4), (12, 0)])
+def test_filter_block_32_shapes(lp, gpu, L, zrun, monkeypatch):
+    """32-vector filter blocks on a lattice that is not a multiple of 4 (one
+    site per wavefront in the FP64 residual kernel) and with and without the
+    y-row work-groups of the FP64 stencils (a 64-column group of the residual
+    spans two 32-vector blocks), against the dense oracle."""
+    tol, nev = 1e-10, 40
+    data = smeared(L, 17, 0.35, 2)
+    monkeypatch.setenv("LAPH_BW", "32")
+    monkeypatch.setenv("LAPH_ZRUN", str(zrun))
+    b = lp.Batch(L, 10, 12.0, 0.0, 2, nev, nsub=96 if L == 9 else 128, tol=tol, seed=5, device=gpu)
+    for t, (ts, _) in enumerate(data):
+        b.set_timeslice(t, ts, 0.62, 0.62, 3)
+    st = b.solve()
+    assert st.filter_block == 32 and st.converged == 2
+    for t, (_, U) in enumerate(data):
+        ev, V, ph, res = b.get(t)
+        check_pairs(ev, V, res, N.dense_D(U.reshape(L, L, L, 3, 3, 3)), nev, tol)
     b.close()
 
 
@@ -358,4 +380,25 @@ def test_batch_solved_twice(lp, gpu):
             check_pairs(ev, V, res, D, nev, tol)
         for a, c in zip(runs[0][t], runs[1][t]):
             np.testing.assert_array_equal(a, c)
+    b.close()
+
+
+@pytest.mark.parametrize("split", ["1", "2", "1,4", "2,2", "2,4"])
+def test_gram_row_blocks_and_site_chunks(lp, gpu, split, monkeypatch):
+    """Every Gram layout -- the full product or its upper triangle in two row
+    blocks, the site sum in 1, 2 or 4 chunks added in order -- gives the dense
+    oracle's spectrum (the Rayleigh quotient's lower triangle is filled from
+    its upper one where full products need it)."""
+    L, nts, nev, tol = 8, 3, 40, 1e-10
+    data = smeared(L, 23, 0.35, nts)
+    monkeypatch.setenv("LAPH_GRAM_SPLIT", split)
+    b = lp.Batch(L, 10, 12.0, 0.0, nts, nev, nsub=128, tol=tol, seed=5, device=gpu)
+    for t, (ts, _) in enumerate(data):
+        b.set_timeslice(t, ts, 0.62, 0.62, 3)
+    st = b.solve()
+    want = [int(x) for x in split.split(",")] + [1]
+    assert st.converged == nts and (st.gram_blocks, st.gram_chunks) == tuple(want[:2])
+    for t, (_, U) in enumerate(data):
+        ev, V, ph, res = b.get(t)
+        check_pairs(ev, V, res, N.dense_D(U.reshape(L, L, L, 3, 3, 3)), nev, tol)
     b.close()
