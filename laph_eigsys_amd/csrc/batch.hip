@@ -75,6 +75,8 @@ static int batch_alloc(laph_batch* b) {
   const size_t nb = (size_t)b->nts * b->per_ts;
   LAPH_HIP(hipMalloc(&b->U64, lk * b->nts * sizeof(double2)));
   LAPH_HIP(hipMalloc(&b->U32, lk * b->nts * sizeof(float2)));
+  LAPH_HIP(hipMalloc(&b->U32c, lk * b->nts * sizeof(float2)));
+  LAPH_HIP(hipMemset(b->U32c, 0, lk * b->nts * sizeof(float2)));
   for (int i = 0; i < 3; ++i) LAPH_HIP(hipMalloc(&b->B[i], nb * sizeof(double2)));
   LAPH_HIP(hipMalloc(&b->R32, nb * b->esz));
   for (int i = 0; i < 3; ++i) LAPH_HIP(hipMalloc(&b->P[i], nb * b->esz));
@@ -155,7 +157,7 @@ static void batch_free(laph_batch* b) {
   if (b->wrb) rocblas_destroy_handle(b->wrb);
   for (void* q : {(void*)b->wV, (void*)b->wG})
     if (q) (void)hipFree(q);
-  void* bufs[] = {b->U64, b->U32, b->dec,  b->Unew, b->B[0], b->B[1],   b->B[2], b->R32,     b->P[0],
+  void* bufs[] = {b->U64, b->U32, b->U32c, b->dec,  b->Unew, b->B[0], b->B[1],   b->B[2], b->R32,     b->P[0],
                   b->P[1], b->P[2], b->G,  b->G2,   b->P2,   b->G3,   b->mu,     b->E,    b->info,    b->nrm2,
                   b->gam, b->coef, b->cab, b->first_blk, b->lock_blk, b->cn, b->coef2, b->jsweeps,
                   b->red, b->phase, b->Gpart};
@@ -832,10 +834,31 @@ static int upload_coefficients(const View& v, const std::vector<double>& lc, int
       }
     }
   };
+  // the kernels read w = a1 (a_t - 6 c_t) + a3 in the last slot (their links
+  // are prescaled by c_t): folded per timeslice after its entries are made
+  auto fold = [&](int tb, int te) {
+    for (int t = tb; t < te; ++t) {
+      const double cy = cab[t].y;
+      for (int st = 0; st + 1 < m; ++st)
+        for (int j = 0; j < K; ++j) {
+          float4& e = co[((size_t)st * nt + t) * K + j];
+          e.w = (float)((double)e.x * cy + (double)e.w);
+        }
+      for (size_t st = 0; st < co2.size() / ((size_t)nt * K); ++st)
+        for (int j = 0; j < K; ++j) {
+          float4& e = co2[(st * nt + t) * K + j];
+          e.w = (float)((double)e.x * cy + (double)e.w);
+        }
+    }
+  };
+  auto fill_fold = [&](int tb, int te) {
+    fill(tb, te);
+    fold(tb, te);
+  };
   const int nth = std::max(1, std::min({8, nt, (int)std::thread::hardware_concurrency()}));
   std::vector<std::thread> pool;
-  for (int h = 1; h < nth; ++h) pool.emplace_back(fill, h * nt / nth, (h + 1) * nt / nth);
-  fill(0, nt / nth);
+  for (int h = 1; h < nth; ++h) pool.emplace_back(fill_fold, h * nt / nth, (h + 1) * nt / nth);
+  fill_fold(0, nt / nth);
   for (auto& th : pool) th.join();
   // coefficient table of the lane: [step][lane timeslice][K] at offset step*nts*K + t0*K
   for (int st = 1; st < m; ++st)
@@ -862,7 +885,7 @@ static int enqueue_filter(const View& v) {
     const void* in = st == 1 ? R : fb(b, b->P[(st - 2) % nb], vo);
     const void* old = st >= 3 && !b->prod ? fb(b, b->P[(st - 3) % 3], vo) : nullptr;
     void* out = fb(b, b->P[(st - 1) % nb], vo);
-    LAPH_HIP(launch_filter_step(b->U32 + v.lo(), in, b->qform ? nullptr : R, old, out,
+    LAPH_HIP(launch_filter_step(b->U32c + v.lo(), in, b->qform ? nullptr : R, old, out,
                                 b->coef + (st - 1) * step + v.nk(),
                                 b->cab + v.ln->t0, b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, b->strip,
                                 stencil_zrun(b), v.ln->st));
@@ -939,7 +962,7 @@ static int enqueue_plain_pass(const View& v, int p) {
     }
     const bool last = st == m;
     const float4* co = b->coef2 + (size_t)(last ? m - 1 + p - 2 : st - 1) * step + v.nk();
-    LAPH_HIP(launch_filter_step(b->U32 + v.lo(), in, last ? f : nullptr, old, out, co, b->cab + v.ln->t0,
+    LAPH_HIP(launch_filter_step(b->U32c + v.lo(), in, last ? f : nullptr, old, out, co, b->cab + v.ln->t0,
                                 b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, b->strip, stencil_zrun(b),
                                 v.ln->st));
   }
@@ -970,6 +993,9 @@ static int run_filter(const View& v, int passes, void** w) {
   if (!*g) LAPH_TRY(capture(v, g, [&] { return enqueue_filter(v); }));
   for (int p = 2; p <= passes; ++p)
     if (!ln->gplain[p - 2]) LAPH_TRY(capture(v, &ln->gplain[p - 2], [&] { return enqueue_plain_pass(v, p); }));
+  // the steps' links: c_t U for this cycle's window of every timeslice
+  LAPH_HIP(launch_scale_links32(v.b->U32 + v.lo(), v.b->cab + ln->t0, v.b->U32c + v.lo(), (size_t)v.b->V3 * 27,
+                                ln->nt, ln->st));
   LAPH_HIP(hipGraphLaunch(*g, ln->st));
   void *f, *q[3];
   plain_buffers(v, &f, q);

@@ -58,6 +58,7 @@ struct laph_batch {
   // links
   double2* U64 = nullptr;  // [nts][V3*27]
   float2* U32 = nullptr;
+  float2* U32c = nullptr;   // [nts][V3*27] the filter steps' links: c_t U32 for the current window
   double2* dec = nullptr;   // HYP scratch for one timeslice
   double2* Unew = nullptr;
   // blocks [nts][n][K]; after a solve the eigenvectors are in B[0]

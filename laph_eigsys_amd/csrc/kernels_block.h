@@ -7,14 +7,16 @@
 namespace laph {
 
 // out = a1 (B in) + a3 in + a2 r + b old per column, B = c_t (H - 6) + a_t;
-// coef[ts*K + col] = (a1, a2, b, a3), cab[ts] = (c_t, a_t - 6 c_t); column
-// blocks below first_blk[ts] are skipped.  old and r may be null (plain
-// recurrence without the source term).  A product-form step, a factor
-// a1 (B - z) with a complex shift z, has no old vector: its shift term is
-// (a3 + i b) in with a3 + i b = -a1 z.
-// Filter vectors stored as FP32 complex (bf16 = 0; bw = 64: one vector per
-// lane, 128: two per lane) or as BF16 complex (bf16 = 1; bw = 128 or 256:
-// two or four per lane); buffers block-major with bw-vector blocks.
+// coef[ts*K + col] = (a1, a2, b, w) with w = a1 (a_t - 6 c_t) + a3, U32 the
+// links prescaled by c_t (launch_scale_links32), so out = a1 (c_t H in) +
+// w in + a2 r + b old; column blocks below first_blk[ts] are skipped.  old
+// and r may be null (plain recurrence without the source term).  A
+// product-form step, a factor a1 (B - z) with a complex shift z, has no old
+// vector: its shift term is (a3 + i b) in with a3 + i b = -a1 z.
+// Filter vectors stored as FP32 complex (bf16 = 0; bw = 32: two sites per
+// wavefront, 64: one vector per lane, 128: two per lane) or as BF16 complex
+// (bf16 = 1; bw = 128 or 256: two or four per lane); buffers block-major with
+// bw-vector blocks.
 // zrun = 1, 2, 4 or 8 (FP32, bw = 128, zrun dividing L, W a multiple of 4):
 // the 4 wavefronts of a work-group take 4 neighbouring y-rows and walk zrun
 // z-sites each, sharing rows through LDS and registers; 0 = one site per
@@ -60,6 +62,9 @@ hipError_t launch_source_shape(const double2* V, int L, int nev, double* psi, hi
 hipError_t launch_normalize_columns(double2* Y, double* tmp, double* red, int n, int K, int nts, int first_col,
                                     hipStream_t st);
 hipError_t launch_convert_links32(const double2* U, float2* U32, size_t n, hipStream_t st);
+// U32c = c_t U32 per timeslice (c_t = cab[ts].x): the filter steps' links.
+hipError_t launch_scale_links32(const float2* U32, const float2* cab, float2* U32c, size_t per_ts, int nts,
+                                hipStream_t st);
 hipError_t launch_random_block(double2* X, size_t total, unsigned long long seed, hipStream_t st);
 // Gaussian columns [first_col, K) of the timeslices with flag[ts] != 0.
 hipError_t launch_reseed_cols(double2* X, int n, int K, int nts, int first_col, const int* flag,

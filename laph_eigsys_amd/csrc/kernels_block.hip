@@ -144,7 +144,12 @@ __device__ __forceinline__ void hop(const C* __restrict__ U, const C* __restrict
 
 // One Chebyshev step in FP32 for all timeslices:
 //   out = a1 * (B in) + a3 * in + a2 * r + b * old,   B = c_t (H - 6) + a_t
-// with per-column coefficients (a1, a2, b, a3) for this step; HAS_R = false
+// with per-column coefficients (a1, a2, b, a3) for this step, evaluated as
+//   out = a1 * (H' in) + w * in + a2 * r + b * old
+// from links prescaled by c_t (H' = c_t H: the filter's link copy is
+// rewritten each cycle) and w = a1 (a_t - 6 c_t) + a3 folded into the
+// coefficient table, so each output component costs one multiply and two
+// FMAs after the hopping sum.  HAS_R = false
 // is the plain recurrence (no source term, one vector stream less), and a
 // null old the product-form step (no previous vector), whose shift term is
 // complex: a3 + i b.
@@ -174,16 +179,12 @@ __global__ __launch_bounds__(256) void k_filter_step32(const float2* __restrict_
   float2 acc[3], ctr[3];
   hop<float2>(Ut, in + vo, s, L, 64, l64, acc, ctr);
   const float4 cf = coef[(size_t)ts * K + lane];
-  const float2 ca = cab[ts];  // (c_t, a_t - 6 c_t)
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const size_t idx = vo + (size_t)(3 * s + c) * 64 + l64;
-    float2 bv;
-    bv.x = ca.x * acc[c].x + ca.y * ctr[c].x;
-    bv.y = ca.x * acc[c].y + ca.y * ctr[c].y;
     float2 o;
-    o.x = cf.x * bv.x + cf.w * ctr[c].x;
-    o.y = cf.x * bv.y + cf.w * ctr[c].y;
+    o.x = cf.x * acc[c].x + cf.w * ctr[c].x;
+    o.y = cf.x * acc[c].y + cf.w * ctr[c].y;
     if (HAS_R) {
       const float2 rv = ld_nt(r + idx);
       o.x += cf.y * rv.x;
@@ -230,13 +231,12 @@ __global__ __launch_bounds__(256) void k_filter_step32h(const float2* __restrict
   float2 acc[3], ctr[3];
   hop<float2>(Ut, in + vo, s, L, 32, l32, acc, ctr);
   const float4 cf = coef[(size_t)ts * K + col];
-  const float2 ca = cab[ts];  // (c_t, a_t - 6 c_t)
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const size_t idx = vo + (size_t)(3 * s + c) * 32 + l32;
     float2 o;
-    o.x = cf.x * (ca.x * acc[c].x + ca.y * ctr[c].x) + cf.w * ctr[c].x;
-    o.y = cf.x * (ca.x * acc[c].y + ca.y * ctr[c].y) + cf.w * ctr[c].y;
+    o.x = cf.x * acc[c].x + cf.w * ctr[c].x;
+    o.y = cf.x * acc[c].y + cf.w * ctr[c].y;
     if (HAS_R) {
       const float2 rv = ld_nt(r + idx);
       o.x += cf.y * rv.x;
@@ -318,15 +318,14 @@ __global__ __launch_bounds__(256) void k_filter_step32x2(const float2* __restric
     }
   }
   const float4 cf0 = coef[(size_t)ts * K + col], cf1 = coef[(size_t)ts * K + col + 1];
-  const float2 ca = cab[ts];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const size_t idx = vo + (size_t)(3 * s + c) * 64 + l64;
     float4 o;
-    o.x = cf0.x * (ca.x * acc0[c].x + ca.y * ctr[c].x) + cf0.w * ctr[c].x;
-    o.y = cf0.x * (ca.x * acc0[c].y + ca.y * ctr[c].y) + cf0.w * ctr[c].y;
-    o.z = cf1.x * (ca.x * acc1[c].x + ca.y * ctr[c].z) + cf1.w * ctr[c].z;
-    o.w = cf1.x * (ca.x * acc1[c].y + ca.y * ctr[c].w) + cf1.w * ctr[c].w;
+    o.x = cf0.x * acc0[c].x + cf0.w * ctr[c].x;
+    o.y = cf0.x * acc0[c].y + cf0.w * ctr[c].y;
+    o.z = cf1.x * acc1[c].x + cf1.w * ctr[c].z;
+    o.w = cf1.x * acc1[c].y + cf1.w * ctr[c].w;
     if (HAS_R) {
       const float4 rv = ld_nt4(r + idx);
       o.x += cf0.y * rv.x;
@@ -425,7 +424,6 @@ __global__ __launch_bounds__(256) void k_filter_step32x2y(const float2* __restri
   const int yup = y == L - 1 ? line - (L - 1) * L : line + L;
   const int ydn = y == 0 ? line + (L - 1) * L : line - L;
   const float4* v = in + vo;
-  const float2 ca = cab[ts];
   const float4 cf0 = coef[(size_t)ts * K + col], cf1 = coef[(size_t)ts * K + col + 1];
   float4 zm[3], zc[3];
   {
@@ -484,10 +482,10 @@ __global__ __launch_bounds__(256) void k_filter_step32x2y(const float2* __restri
       for (int c = 0; c < 3; ++c) {
         const unsigned ro = (unsigned)((3 * su + c) * 64);
         const float4 ctr = zc[c];
-        o[c].x = cf0.x * (ca.x * acc0[c].x + ca.y * ctr.x) + cf0.w * ctr.x;
-        o[c].y = cf0.x * (ca.x * acc0[c].y + ca.y * ctr.y) + cf0.w * ctr.y;
-        o[c].z = cf1.x * (ca.x * acc1[c].x + ca.y * ctr.z) + cf1.w * ctr.z;
-        o[c].w = cf1.x * (ca.x * acc1[c].y + ca.y * ctr.w) + cf1.w * ctr.w;
+        o[c].x = cf0.x * acc0[c].x + cf0.w * ctr.x;
+        o[c].y = cf0.x * acc0[c].y + cf0.w * ctr.y;
+        o[c].z = cf1.x * acc1[c].x + cf1.w * ctr.z;
+        o[c].w = cf1.x * acc1[c].y + cf1.w * ctr.w;
         if (HAS_R) {
           const float4 rv = ld_nt4(r + vo + ro + l64);
           o[c].x += cf0.y * rv.x;
@@ -611,7 +609,6 @@ __global__ __launch_bounds__(256) void k_filter_step_bf16(const float2* __restri
   float4 cf[VPL];
 #pragma unroll
   for (int j = 0; j < VPL; ++j) cf[j] = coef[(size_t)ts * K + col + j];
-  const float2 ca = cab[ts];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const size_t idx = vo + (size_t)(3 * s + c) * 64 + l64;
@@ -621,8 +618,7 @@ __global__ __launch_bounds__(256) void k_filter_step_bf16(const float2* __restri
 #pragma unroll
     for (int j = 0; j < VPL; ++j) {
       const float2 xc = bf2_unpack(ctr[c][j]);
-      float2 val = make_float2(cf[j].x * (ca.x * acc[c][j].x + ca.y * xc.x) + cf[j].w * xc.x,
-                               cf[j].x * (ca.x * acc[c][j].y + ca.y * xc.y) + cf[j].w * xc.y);
+      float2 val = make_float2(cf[j].x * acc[c][j].x + cf[j].w * xc.x, cf[j].x * acc[c][j].y + cf[j].w * xc.y);
       if (HAS_R) {
         const float2 rr = bf2_unpack(rv[j]);
         val.x += cf[j].y * rr.x;
@@ -947,6 +943,24 @@ __global__ void k_extract_cols(const double2* __restrict__ X, double2* __restric
   const int col = blockIdx.y;
   for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < n; row += gridDim.x * blockDim.x)
     V[(size_t)col * n + row] = X[(size_t)row * K + col];
+}
+
+// U32c[ts][i] = c_t U32[ts][i], c_t = cab[ts].x: the filter's link copy for the
+// current window of every timeslice of a lane (per_ts link entries each).
+__global__ void k_scale_links32(const float2* __restrict__ U32, const float2* __restrict__ cab,
+                                float2* __restrict__ U32c, size_t per_ts) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= per_ts) return;
+  const size_t e = (size_t)blockIdx.y * per_ts + i;
+  const float c = cab[blockIdx.y].x;
+  const float2 u = U32[e];
+  U32c[e] = make_float2(c * u.x, c * u.y);
+}
+
+hipError_t launch_scale_links32(const float2* U32, const float2* cab, float2* U32c, size_t per_ts, int nts,
+                                hipStream_t st) {
+  k_scale_links32<<<dim3((unsigned)((per_ts + 255) / 256), nts), 256, 0, st>>>(U32, cab, U32c, per_ts);
+  return hipGetLastError();
 }
 
 __global__ void k_convert_links32(const double2* __restrict__ U, float2* __restrict__ U32, size_t n) {
