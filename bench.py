@@ -51,8 +51,8 @@ PEAK_FP64_MFMA = 78.6e12  # dense FP64 matrix peak of the MI355X
 # 1 GiB device copy (python bench.py --summarize-pmc ...), with the sha256
 # (first 16 hex digits) of the source file of the kernel it was measured on:
 # the figure is reported only while that file is unchanged.
-TRAFFIC = {"bytes_per_launch": 43334465271, "source": "laph_eigsys_amd/csrc/kernels_block.hip",
-           "sha16": "0000000000000000"}
+TRAFFIC = {"bytes_per_launch": 43532763820, "source": "laph_eigsys_amd/csrc/kernels_block.hip",
+           "sha16": "f66f0d46e1eb700f"}  # profiles/filter_step_32x32x32x64_k256_pmc.md
 
 
 def main():
@@ -67,6 +67,8 @@ def main():
                     help="only the CPU baseline as planned: one timeslice per host core, each solved to 1e-10 by "
                          "the oracle (single-threaded), --cpu-runs times")
     ap.add_argument("--cpu-cores", type=int, default=0, help="with --cpu-full: cores (0 = OMP_NUM_THREADS or all)")
+    ap.add_argument("--cpu-threads-per-ts", type=int, default=1,
+                    help="with --cpu-full: OpenMP threads per timeslice (ranks = cores / threads)")
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--cpu-worker", type=str, default=None, help=argparse.SUPPRESS)
     ap.add_argument("--filter-reps", type=int, default=20)
@@ -104,7 +106,7 @@ def main():
         cpu_worker(args.cpu_worker)
         return
     if args.cpu_full:
-        print(json.dumps(cpu_full(args.cpu_cores, args.cpu_runs, args.nts)))
+        print(json.dumps(cpu_full(args.cpu_cores, args.cpu_runs, args.nts, args.cpu_threads_per_ts)))
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); timeslices are "
@@ -631,24 +633,26 @@ def cpu_worker(spec):
     print(json.dumps({"seconds": time.perf_counter() - t0, "nconv": nconv, "matvecs": stats[1]}))
 
 
-def cpu_full(cores, runs, nts):
-    """The CPU baseline as planned: timeslices spread one per host core (the
-    reference's rank-per-timeslice split, ev_ts.cpp:96-126), each solved to
-    1e-10 by the single-threaded oracle; per run, `cores` timeslices start
-    together and the rate is cores / (time until the last finishes)."""
+def cpu_full(cores, runs, nts, threads=1):
+    """The CPU baseline as planned: timeslices spread over the host cores
+    like the reference's ranks (ev_ts.cpp:96-126), `threads` OpenMP threads
+    each, every one solved to 1e-10 by the oracle; per run, cores/threads
+    timeslices start together and the rate is their count / the time until
+    the last finishes (process start and HYP included)."""
     import subprocess
     import tempfile
     from oracle import lapheig_np as N
     cores = cores or int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    ranks = max(1, min(nts, cores // max(1, threads)))
     lam_c = plain_lambda_c(L, NEV)
     vol = 72 * L ** 3
-    cfg = N.synth_config(SEED, EPS, L, min(nts, cores))
+    cfg = N.synth_config(SEED, EPS, L, ranks)
     tmp = tempfile.mkdtemp(prefix="laph_cpu_", dir=os.environ.get("TMPDIR", "/tmp"))
-    env = dict(os.environ, OMP_NUM_THREADS="1")
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
     rates, per_ts = [], []
     try:
         paths = []
-        for t in range(min(nts, cores)):
+        for t in range(ranks):
             p = os.path.join(tmp, f"ts{t}.npy")
             np.save(p, cfg[t * vol:(t + 1) * vol])
             paths.append(p)
@@ -656,20 +660,27 @@ def cpu_full(cores, runs, nts):
             t0 = time.perf_counter()
             procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker", f"{p},{lam_c}"],
                                       stdout=subprocess.PIPE, text=True, env=env) for p in paths]
-            res = [json.loads(pr.communicate()[0]) for pr in procs]
+            res, left = [None] * len(procs), set(range(len(procs)))
+            while left:  # report each timeslice as it finishes
+                for k in sorted(left):
+                    if procs[k].poll() is not None:
+                        res[k] = json.loads(procs[k].communicate()[0])
+                        left.discard(k)
+                        print(f"[bench] cpu run {r}: timeslice {k} solved in {res[k]['seconds']:.1f}s "
+                              f"({time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
+                time.sleep(0.2)
             wall = time.perf_counter() - t0
             assert all(x["nconv"] >= NEV for x in res), res
             rates.append(len(paths) / wall)
             per_ts += [x["seconds"] for x in res]
-            print(f"[bench] cpu run {r}: {len(paths)} timeslices on {len(paths)} cores in {wall:.1f}s",
-                  file=sys.stderr, flush=True)
+            print(f"[bench] cpu run {r}: {len(paths)} timeslices in {wall:.1f}s", file=sys.stderr, flush=True)
     finally:
         import shutil
         shutil.rmtree(tmp, ignore_errors=True)
-    return {"value": float(np.median(rates)), "unit": "timeslices/s", "cores": len(paths), "kind": "port",
-            "sample": f"{len(paths)} timeslices of the 32^3x64 field, one per core (single-threaded oracle "
-                      f"Krylov-Schur to {TOL}, lambda_c {lam_c:.4f}), median rate of {runs} runs (process start and "
-                      "HYP included in each run's wall time)",
+    return {"value": float(np.median(rates)), "unit": "timeslices/s", "cores": ranks * threads, "kind": "port",
+            "sample": f"{ranks} timeslices of the 32^3x64 field at once, {threads} OpenMP thread(s) each (the "
+                      f"oracle's Krylov-Schur to {TOL}, lambda_c {lam_c:.4f}), median rate of {runs} runs (process "
+                      "start and HYP included in each run's wall time)",
             "rates": rates, "seconds_per_timeslice": spread(per_ts), **host_info()}
 
 

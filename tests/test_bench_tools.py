@@ -84,3 +84,31 @@ def test_trace_phases(bench, tmp_path, capsys):
     filt = [ln for ln in out.splitlines() if ln.strip().startswith("filter")][0]
     assert "4.0 ms wall" in filt and "1 dispatches" in filt
     assert "large GEMMs" in out and "rocSOLVER" in out
+
+
+def test_plain_lambda_c_is_above_the_wanted_free_levels(bench):
+    """lambda_c of the plain drop-in: 1.35 x the nev-th free-field level
+    (levels 4 sum_i sin^2(pi k_i / L), each three-fold)."""
+    import numpy as np
+    lam = bench.plain_lambda_c(32, 220)
+    s2 = 4 * np.sin(np.pi * np.arange(32) / 32) ** 2
+    lev = np.sort((s2[:, None, None] + s2[None, :, None] + s2[None, None, :]).ravel())
+    counts = np.repeat(lev, 3)
+    assert abs(lam - 1.35 * counts[219]) < 1e-15
+    assert counts[219] < lam < 12.0
+
+
+def test_spread_and_traffic_tie(bench, monkeypatch):
+    assert bench.spread([]) is None
+    assert bench.spread([3.0, 1.0, 2.0]) == {"min": 1.0, "median": 2.0, "max": 3.0}
+    import hashlib
+    src = open(os.path.join(ROOT, bench.TRAFFIC["source"]), "rb").read()
+    monkeypatch.setitem(bench.TRAFFIC, "sha16", hashlib.sha256(src).hexdigest()[:16])
+    assert bench.traffic_for_build() == bench.TRAFFIC["bytes_per_launch"]
+    monkeypatch.setitem(bench.TRAFFIC, "sha16", "0" * 16)
+    assert bench.traffic_for_build() is None
+
+
+def test_host_info(bench):
+    h = bench.host_info()
+    assert h["nproc"] == os.cpu_count()
