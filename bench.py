@@ -170,7 +170,7 @@ def main():
     ev_batch = [b.get(t, want_vectors=False)[0] for t in range(nts)]
     # dominant kernel: FP32 filter stencil, timed alone on the device
     ms_launch, bytes_launch = b.time_filter(reps=args.filter_reps)
-    gram = gram_rate(b)
+    gram = gram_rate(b, st)
     b.close()
     achieved = bytes_launch / (ms_launch * 1e-3)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
@@ -487,8 +487,7 @@ def workload_c4(lp, args):
         print(f"[bench] c4 group {g0}..{g0 + n - 1}: {st.seconds:.2f}s cycles={st.iterations}", file=sys.stderr,
               flush=True)
     ms, by = b.time_filter(reps=5)
-    gram = gram_rate(b)
-    gram["blocks"] = st.gram_blocks
+    gram = gram_rate(b, st)
     # streaming device -> file of two timeslices' eigensystems (1.06 GB of
     # eigenvectors each) into a scratch directory that is removed afterwards
     import shutil
@@ -548,13 +547,15 @@ def device_sync():
     return sync
 
 
-def gram_rate(b, reps=10):
+def gram_rate(b, st, reps=10):
     """The FP64 matrix-core kernel of the dense phase: the Gram product of the
     first lane's timeslices at full width, timed alone on the device."""
     ms, flops = b.time_gram(reps=reps)
     rate = flops / (ms * 1e-3)
-    return {"kernel": "strided-batched zgemm, Hermitian upper-triangle row blocks", "ms": ms,
-            "tflop_executed": flops / 1e12, "TFLOPps": rate / 1e12, "frac_fp64_mfma_peak": rate / PEAK_FP64_MFMA}
+    kern = ("k_gram_mfma (hand-written, v_mfma_f64_16x16x4, 64x64 upper tiles)" if st.gram_mfma else
+            f"rocBLAS strided-batched zgemm, {st.gram_blocks} row block(s)")
+    return {"kernel": kern + f", {st.gram_chunks} site chunk(s)", "ms": ms, "tflop_executed": flops / 1e12,
+            "TFLOPps": rate / 1e12, "frac_fp64_mfma_peak": rate / PEAK_FP64_MFMA}
 
 
 def plain_lambda_c(L, nev, factor=1.35):

@@ -213,8 +213,10 @@ typedef struct laph_batch laph_batch;
  *                           (else its explicit inverse and GEMMs)
  *   LAPH_EIG=0|1|2          Rayleigh-Ritz eigensolver: zheevd, zheevdj, zheevj (0)
  *   LAPH_GRAM_SPLIT=s[,c]   row blocks s of the Hermitian Gram and Rayleigh-quotient products (upper
- *                           triangle only; 1: the full products) and site chunks c (1, 2, 4) of their
- *                           sums (else both timed on the first solve)
+ *                           triangle only; 1: the full products) and site chunks c (1, 2, 4, 8) of their
+ *                           sums (else both timed on the first solve, with the kernel)
+ *   LAPH_GRAM_MFMA=0|1      Gram products on rocBLAS zgemm (0) or on the hand-written FP64 MFMA
+ *                           kernel (1); else timed on the first solve
  *   LAPH_TRACE=1            per-cycle trace of the first timeslice on stderr
  *   LAPH_NET_TARGET=1       laph_net_train_step fits the span of all K Ritz vectors of the last solve
  *                           (else its nev eigenvectors)
@@ -272,8 +274,10 @@ typedef struct laph_batch_stats {
                               positive definite) */
   int gram_blocks;         /* row blocks of the Hermitian Gram products (1: full zgemm, 2: upper
                               triangle in two row blocks), timed on the first solve */
-  int gram_chunks;         /* site chunks of the Gram products (1, 2 or 4: partial products as
+  int gram_chunks;         /* site chunks of the Gram products (1, 2, 4 or 8: partial products as
                               extra batch entries, added in order), timed with gram_blocks */
+  int gram_mfma;           /* 1: the Gram products ran on the hand-written FP64 MFMA kernel (the
+                              64 x 64 tiles of the upper triangle), 0: on rocBLAS zgemm */
 } laph_batch_stats;
 
 int laph_batch_create(laph_batch** out, const laph_params* params, const laph_batch_opts* opts);

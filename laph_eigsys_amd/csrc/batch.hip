@@ -271,10 +271,19 @@ __global__ void k_sum_gram_chunks(const double2* __restrict__ P, double2* __rest
 static int gram(const View& v, Act q, const double2* A, const double2* Bm, double2* Gt, double alpha) {
   const int K = v.b->K;
   const cplx al(alpha, 0.0), zero(0.0, 0.0);
-  const int bs = gram_rows(q.a, std::max(1, v.b->gram_split));
   const int S = v.b->gram_ksplit;
-  const int rows = v.b->n / S;  // S divides n
   double2* C = S > 1 ? v.b->Gpart + (size_t)v.ln->t0 * S * K * K : Gt;
+  if (v.b->gram_mfma && gram_mfma_fits(v.b->n, q.a, S)) {  // hand-written: the 64 x 64 upper tiles
+    LAPH_HIP(launch_gram_mfma(A, Bm, C, v.b->n, K, q.l, q.a, v.ln->nt, S, v.b->per_ts, alpha, v.ln->st));
+    if (S > 1) {
+      k_sum_gram_chunks<<<dim3((unsigned)((q.a * q.a + 255) / 256), v.ln->nt), 256, 0, v.ln->st>>>(C, Gt, q.a, K, S,
+                                                                                                     64);
+      LAPH_HIP(hipGetLastError());
+    }
+    return LAPH_OK;
+  }
+  const int bs = gram_rows(q.a, std::max(1, v.b->gram_split));
+  const int rows = v.b->n / S;  // S divides n
   for (int r0 = 0; r0 < q.a; r0 += bs) {
     const int m = std::min(bs, q.a - r0);
     LAPH_RB(rocblas_zgemm_strided_batched(v.ln->rb, rocblas_operation_none, rocblas_operation_conjugate_transpose, m,
@@ -305,6 +314,10 @@ __global__ void k_herm_fill(double2* __restrict__ G, int a, int K) {
 // Floating-point operations the Gram of one lane at active width a executes
 // (complex multiply-adds of the row blocks computed, 8 flops each).
 static double gram_flops(const laph_batch* b, int a, int nt) {
+  if (b->gram_mfma && gram_mfma_fits(b->n, a, b->gram_ksplit)) {
+    const int T = a / 64;
+    return 8.0 * b->n * 64.0 * 64.0 * (T * (T + 1) / 2) * nt;
+  }
   const int bs = gram_rows(a, std::max(1, b->gram_split));
   double f = 0;
   for (int r0 = 0; r0 < a; r0 += bs) f += 8.0 * b->n * (double)std::min(bs, a - r0) * (a - r0);
@@ -460,7 +473,7 @@ static int rayleigh_ritz(const View& v, Act q, bool orthonormal, bool* chol_fail
   double* sc = b->cn + v.nk();
   const dim3 gs((unsigned)((q.a + 255) / 256), nt), ga((unsigned)((q.a * q.a + 255) / 256), nt);
   if (!orthonormal) {
-    if (b->gram_split > 1) {  // Mt enters full products below
+    if (b->gram_split > 1 || b->gram_mfma) {  // Mt (upper triangle) enters full products below
       k_herm_fill<<<ga, 256, 0, v.ln->st>>>(G, q.a, K);
       LAPH_HIP(hipGetLastError());
     }
@@ -1262,11 +1275,14 @@ static int autotune_block_width(laph_batch* b) {
     const View v{b, ln};
     const Act q{0, b->K};
     float best_ms = 0;
-    int best_split = 1, best_ks = 1;
+    int best_split = 1, best_ks = 1, best_mfma = 0;
     bool first_g = true;
-    for (int split : {1, 2})
-      for (int ks : {1, 2, 4}) {
+    for (int mf : {0, 1})
+      for (int split : {1, 2})
+      for (int ks : {1, 2, 4, 8}) {
         if (b->n % ks != 0 || ks > b->gram_ksplit_max) continue;
+        if (mf && (split > 1 || !gram_mfma_fits(b->n, b->K, ks))) continue;  // one layout of the MFMA kernel
+        b->gram_mfma = mf;
         b->gram_split = split;
         b->gram_ksplit = ks;
         LAPH_TRY(gram(v, q, v.X(), v.X(), b->G2 + v.kk(), 1.0));  // warm-up
@@ -1276,17 +1292,18 @@ static int autotune_block_width(laph_batch* b) {
         LAPH_HIP(hipEventSynchronize(ln->ev[1]));
         float ms = 0;
         LAPH_HIP(hipEventElapsedTime(&ms, ln->ev[0], ln->ev[1]));
-        if (first_g || ms < best_ms) best_ms = ms, best_split = split, best_ks = ks;
+        if (first_g || ms < best_ms) best_ms = ms, best_split = split, best_ks = ks, best_mfma = mf;
         first_g = false;
       }
     b->gram_split = best_split;
     b->gram_ksplit = best_ks;
+    b->gram_mfma = best_mfma;
   }
   LAPH_HIP(hipGraphExecDestroy(ln->graph));  // captured for the last layout timed
   ln->graph = nullptr;
   if (b->trace)
-    std::fprintf(stderr, "[laph] filter block width %d, strip %d, Gram row blocks %d, site chunks %d\n", wbest, sbest,
-                 b->gram_split, b->gram_ksplit);
+    std::fprintf(stderr, "[laph] filter block width %d, strip %d, Gram %s, row blocks %d, site chunks %d\n", wbest,
+                 sbest, b->gram_mfma ? "MFMA kernel" : "rocBLAS", b->gram_split, b->gram_ksplit);
   return LAPH_OK;
 }
 
@@ -1350,6 +1367,7 @@ static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
   s.filter_block = b->bw;
   s.gram_blocks = b->gram_split;
   s.gram_chunks = b->gram_ksplit;
+  s.gram_mfma = b->gram_mfma;
   s.filter_strip = b->strip;
   s.filter_zrun = b->bf16 == 0 && b->bw == 128 ? laph::stencil_zrun(b) : 0;
   s.start_window = b->window0;
@@ -1458,6 +1476,10 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
       const char* c = std::strchr(e, ',');
       b->gram_ksplit = c ? std::max(1, std::min(b->gram_ksplit_max, std::atoi(c + 1))) : 1;
       if (b->n % b->gram_ksplit != 0) b->gram_ksplit = 1;
+      b->gram_split_fixed = true;
+    }
+    if (const char* e = std::getenv("LAPH_GRAM_MFMA")) {
+      b->gram_mfma = std::atoi(e) != 0;
       b->gram_split_fixed = true;
     }
     if (const char* e = std::getenv("LAPH_NET_TARGET")) b->net_target_all = std::atoi(e) != 0;

@@ -132,8 +132,9 @@ struct laph_batch {
   int gram_split = 2;        // row blocks of the Hermitian Gram / Rayleigh-quotient products, 1 (the full
                              // product) or 2, timed on the first solve (LAPH_GRAM_SPLIT fixes it)
   bool gram_split_fixed = false;
-  int gram_ksplit = 1;       // site chunks of the Gram products (1, 2 or 4), timed with gram_split
-  int gram_ksplit_max = 4;
+  int gram_ksplit = 1;       // site chunks of the Gram products (1, 2, 4 or 8), timed with gram_split
+  int gram_mfma = 0;         // 1: the hand-written FP64 MFMA kernel (64 x 64 upper tiles) instead of rocBLAS
+  int gram_ksplit_max = 8;
   double2* Gpart = nullptr;  // [nts][gram_ksplit_max][K*K] partial Gram products of the site chunks
 };
 

@@ -62,6 +62,13 @@ hipError_t launch_source_shape(const double2* V, int L, int nev, double* psi, hi
 hipError_t launch_normalize_columns(double2* Y, double* tmp, double* red, int n, int K, int nts, int first_col,
                                     hipStream_t st);
 hipError_t launch_convert_links32(const double2* U, float2* U32, size_t n, hipStream_t st);
+// C[ts*S + c] = alpha sum over site chunk c of A[s][l+i] conj(B[s][l+j]) on
+// the 64x64 tiles ti <= tj of the leading a x a block (column-major, lda K),
+// FP64 matrix cores (gram_mfma.hip).  Needs a % 64 == 0 and (n / S) % 16 == 0
+// (gram_mfma_fits).
+bool gram_mfma_fits(int n, int a, int S);
+hipError_t launch_gram_mfma(const double2* A, const double2* B, double2* C, int n, int K, int l, int a, int nts,
+                            int S, size_t per_ts, double alpha, hipStream_t st);
 // U32c = c_t U32 per timeslice (c_t = cab[ts].x): the filter steps' links.
 hipError_t launch_scale_links32(const float2* U32, const float2* cab, float2* U32c, size_t per_ts, int nts,
                                 hipStream_t st);

@@ -96,12 +96,13 @@ class IO:
 
 def batch_bytes_per_timeslice(L, nev, nsub=0):
     """Device bytes of one timeslice in a batch: three FP64 blocks, four FP32
-    filter buffers, eight K x K FP64 matrices (Gram, Rayleigh quotient, the
-    inverse Cholesky factor, scratch, four partial Grams), FP64 + FP32 links."""
+    filter buffers, twelve K x K FP64 matrices (Gram, Rayleigh quotient, the
+    inverse Cholesky factor, scratch, eight partial Grams), FP64 links and two
+    FP32 copies (as read, and scaled for the filter's window)."""
     n = 3 * L ** 3
     K = nsub if nsub > 0 else nev + max(16, nev // 4)
     K = (K + 63) // 64 * 64
-    return n * K * (3 * 16 + 4 * 8) + 8 * K * K * 16 + L ** 3 * 27 * 24
+    return n * K * (3 * 16 + 4 * 8) + 12 * K * K * 16 + L ** 3 * 27 * 36
 
 
 def plan_groups(todo, L, nev, mem_gb, group=0, nsub=0):

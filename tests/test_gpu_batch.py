@@ -383,21 +383,24 @@ def test_batch_solved_twice(lp, gpu):
     b.close()
 
 
-@pytest.mark.parametrize("split", ["1", "2", "1,4", "2,2", "2,4"])
-def test_gram_row_blocks_and_site_chunks(lp, gpu, split, monkeypatch):
-    """Every Gram layout -- the full product or its upper triangle in two row
-    blocks, the site sum in 1, 2 or 4 chunks added in order -- gives the dense
-    oracle's spectrum (the Rayleigh quotient's lower triangle is filled from
-    its upper one where full products need it)."""
+@pytest.mark.parametrize("split,mfma", [("1", "0"), ("2", "0"), ("1,4", "0"), ("2,2", "0"), ("2,4", "0"),
+                                        ("1", "1"), ("1,2", "1"), ("1,4", "1"), ("1,8", "1")])
+def test_gram_row_blocks_and_site_chunks(lp, gpu, split, mfma, monkeypatch):
+    """Every Gram layout -- rocBLAS zgemm on the full product or its upper
+    triangle in two row blocks, or the hand-written FP64 MFMA kernel on the
+    64 x 64 upper tiles; the site sum in 1, 2 or 4 chunks added in order --
+    gives the dense oracle's spectrum (the Rayleigh quotient's lower triangle
+    is filled from its upper one where full products need it)."""
     L, nts, nev, tol = 8, 3, 40, 1e-10
     data = smeared(L, 23, 0.35, nts)
     monkeypatch.setenv("LAPH_GRAM_SPLIT", split)
+    monkeypatch.setenv("LAPH_GRAM_MFMA", mfma)
     b = lp.Batch(L, 10, 12.0, 0.0, nts, nev, nsub=128, tol=tol, seed=5, device=gpu)
     for t, (ts, _) in enumerate(data):
         b.set_timeslice(t, ts, 0.62, 0.62, 3)
     st = b.solve()
     want = [int(x) for x in split.split(",")] + [1]
-    assert st.converged == nts and (st.gram_blocks, st.gram_chunks) == tuple(want[:2])
+    assert st.converged == nts and (st.gram_blocks, st.gram_chunks, st.gram_mfma) == (want[0], want[1], int(mfma))
     for t, (_, U) in enumerate(data):
         ev, V, ph, res = b.get(t)
         check_pairs(ev, V, res, N.dense_D(U.reshape(L, L, L, 3, 3, 3)), nev, tol)
