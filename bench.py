@@ -302,7 +302,7 @@ def summarize_trace(db, last):
 def _phase(name):
     if _filter_kernel(name):
         return "filter (FP32 stencil steps)"
-    if name.startswith("Cijk") and ("MT64x64" in name or "MT128x64" in name):
+    if "k_gram_mfma" in name or (name.startswith("Cijk") and ("MT64x64" in name or "MT128x64" in name)):
         return "large GEMMs (Gram, Rayleigh quotient, rotation, projection)"
     if "laph" in name:
         return "other library kernels (FP64 stencil, residual, combine, ...)"

@@ -254,7 +254,7 @@ typedef struct laph_batch_stats {
   long long filter_launches; /* FP32 filter-step kernel launches */
   double filter_bytes;     /* algorithmic bytes of those launches (vectors read/written
                               once, links once per vector block) */
-  int filter_block;        /* vectors per block of the filter (FP32 storage: 64 or 128,
+  int filter_block;        /* vectors per block of the filter (FP32 storage: 32, 64 or 128,
                               BF16: 128 or 256), chosen by timing on the first solve
                               unless LAPH_BW fixes it */
   int filter_strip;        /* sweep strip width of the stencil kernels (L = x-major

@@ -10,10 +10,13 @@ eigenvectors grow by up to A = T_m(theta_max)^P.  Two candidates could push
 the equilibrated Gram's smallest eigenvalue towards the level where an FP64
 Cholesky of K columns fails (about K * 2^-53): the amplification itself
 (columns drawn towards the few most amplified eigenvectors), or the FP32
-rounding of e (about 2^-24 |e| in random directions).  Emulated at 8^3 and
-16^3, both arithmetics give the same Gram spectra in every cycle: the drop
-in the multi-pass cycles comes from the amplification; FP32 rounding adds
-nothing measurable.
+rounding of e (about 2^-24 |e| in random directions).  Emulated at 8^3, both
+arithmetics give the same Gram spectra in every cycle of bounded
+amplification; in the uncapped four-pass cycle (amplification about 2e21)
+the Gram is singular at working precision with either arithmetic (smallest
+eigenvalue 1e-16 with the FP32 filter, -1.6e-15 with the FP64 one, whose
+Cholesky fails): the loss of rank comes from the amplification, and FP32
+rounding adds nothing measurable.
 
 Two filter schedules are emulated:
   "uncapped": two single-pass warm-up cycles of T_16(B) from a Gaussian
@@ -126,8 +129,7 @@ def filter_three_term(U, X, theta, c, a, m, passes, dtype):
         nxt = (2 * s[k] / s[k + 1]).astype(np.float32)[None, :] * B(e) \
             - (s[k - 1] / s[k + 1]).astype(np.float32)[None, :] * e_old \
             + (2 * tau[k] / s[k + 1]).astype(np.float32)[None, :] * r
-        e_old, e = (e * (s[k] / s[k + 1]).astype(np.float32)[None, :]).astype(dtype), nxt.astype(dtype)
-        # e_old is carried at the scale of the new step (s_k / s_{k+1})
+        e_old, e = e, nxt.astype(dtype)  # e_k = E_k / s_k, each at its own scale
     gamma = tau[m] / s[m]
     w1 = e
     w = w1
