@@ -175,7 +175,8 @@ def main():
     achieved = bytes_launch / (ms_launch * 1e-3)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                 "frac": achieved / PEAK_HBM, "traffic": traffic_for_build(), "traffic_unit": "bytes/launch",
-                "kernel": (f"k_filter_step32x2y (block 128, y-row groups walking z-lines in runs of {st.filter_zrun})"
+                "kernel": (f"k_filter_step32{'x2y' if st.filter_block == 128 else 'y'} (block {st.filter_block}, "
+                           f"y-row groups walking z-lines in runs of {st.filter_zrun})"
                            if st.filter_zrun else
                            f"k_filter_step32{'x2' if st.filter_block == 128 else ''} (block {st.filter_block})"),
                 "avg_launch_us": ms_launch * 1e3, "bytes_per_launch": bytes_launch,

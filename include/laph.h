@@ -262,9 +262,9 @@ typedef struct laph_batch_stats {
                               LAPH_STRIP fixes it */
   int recoveries;          /* times the active columns of failing timeslices were re-seeded
                               after even shifted Cholesky QR failed (a non-finite block) */
-  int filter_zrun;         /* FP32 128-vector filter step in work-groups of 4 y-rows: z-sites
-                              per wavefront (8 or 4; LAPH_ZRUN fixes it); 0 = one site per
-                              wavefront (other layouts, L not a multiple of 4, LAPH_ZRUN=0) */
+  int filter_zrun;         /* FP32 64- or 128-vector filter step in work-groups of 4 y-rows:
+                              z-sites per wavefront (8 or 4; LAPH_ZRUN fixes it); 0 = one site
+                              per wavefront (other layouts, L not a multiple of 4, LAPH_ZRUN=0) */
   double start_window;     /* window lower end of the first cycle from the start block: 3 x the
                               K-th level of the free Laplacian, at most 0.75 lambda_l
                               (LAPH_WINDOW0 fixes it) */

@@ -1369,7 +1369,7 @@ static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
   s.gram_chunks = b->gram_ksplit;
   s.gram_mfma = b->gram_mfma;
   s.filter_strip = b->strip;
-  s.filter_zrun = b->bf16 == 0 && b->bw == 128 ? laph::stencil_zrun(b) : 0;
+  s.filter_zrun = b->bf16 == 0 && (b->bw == 128 || b->bw == 64) ? laph::stencil_zrun(b) : 0;
   s.start_window = b->window0;
   s.converged = conv;
   s.max_residual = maxres;

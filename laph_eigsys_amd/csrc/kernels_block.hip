@@ -519,6 +519,133 @@ __global__ __launch_bounds__(256) void k_filter_step32x2y(const float2* __restri
   }
 }
 
+// One vector per lane: link products of one 64-vector block.
+__device__ __forceinline__ void link_terms1(const float2* __restrict__ Uf, const float2* __restrict__ Ub,
+                                            const float2 xu[3], const float2 xm[3], float2 acc[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+#pragma unroll
+    for (int bb = 0; bb < 3; ++bb) {
+      acc[a] = cfma(Uf[3 * a + bb], xu[bb], acc[a]);
+      acc[a] = cfmac(Ub[3 * bb + a], xm[bb], acc[a]);
+    }
+  }
+}
+
+// The y-row work-group step of k_filter_step32x2y on 64-vector blocks: one
+// vector per lane (float2 rows of 512 B), so a wavefront holds half the
+// registers and twice as many wavefronts fit on a SIMD; the links are read
+// once per 64 vectors instead of once per 128.  Same walk, LDS exchange and
+// barrier structure; work-groups of locked blocks return at once.
+template <bool HAS_R, bool OLD, int NZ>
+__global__ __launch_bounds__(256) void k_filter_step32y(const float2* __restrict__ U32,
+                                                        const float2* __restrict__ in,
+                                                        const float2* __restrict__ r,
+                                                        const float2* __restrict__ old,
+                                                        float2* __restrict__ out,
+                                                        const float4* __restrict__ coef,
+                                                        const int* __restrict__ first_blk, int L, int K, int nts,
+                                                        int W) {
+  __shared__ float2 rows[2][4][3][64];
+  const int V3 = L * L * L;
+  const int nyb = W / 4;
+  int ts, cb, g;
+  unit_map(V3 / (4 * L), K / 64, nts, ts, cb, g);
+  if (cb < first_blk[ts]) return;  // whole work-group: locked block
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int yb = g % nyb;
+  const int t = g / nyb;
+  const int x = t % L, strip = t / L;
+  const int y = strip * W + yb * 4 + wave;
+  const int l64 = threadIdx.x & 63;
+  const int col = cb * 64 + l64;
+  const size_t vo = ((size_t)ts * (K / 64) + cb) * (size_t)V3 * 3 * 64;  // in float2 units
+  const float2* Ut = U32 + (size_t)ts * V3 * 27;
+  const int L2 = L * L;
+  const int line = (x * L + y) * L;
+  const int xup = x == L - 1 ? line - (L - 1) * L2 : line + L2;
+  const int xdn = x == 0 ? line + (L - 1) * L2 : line - L2;
+  const int yup = y == L - 1 ? line - (L - 1) * L : line + L;
+  const int ydn = y == 0 ? line + (L - 1) * L : line - L;
+  const float2* v = in + vo;
+  auto rw = [&](int site, int c) { return v + (unsigned)((3 * site + c) * 64); };
+  const float4 cf = coef[(size_t)ts * K + col];
+  float2 zm[3], zc[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    zm[c] = rw(line + L - 1, c)[l64];
+    zc[c] = rw(line, c)[l64];
+  }
+  for (int z0 = 0; z0 < L; z0 += NZ) {
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+      const int z = z0 + k;
+      const int par = z & 1;
+      const int s = line + z;
+      const int zdn = line + (z == 0 ? L - 1 : z - 1);
+      const int zup = line + (z == L - 1 ? 0 : z + 1);
+      float2 zn[3], yu[3], ym[3], xu[3], xm[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) rows[par][wave][c][l64] = zc[c];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        zn[c] = rw(zup, c)[l64];
+        xu[c] = rw(xup + z, c)[l64];
+        xm[c] = rw(xdn + z, c)[l64];
+      }
+      if (wave == 3) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) yu[c] = rw(yup + z, c)[l64];
+      }
+      if (wave == 0) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ym[c] = rw(ydn + z, c)[l64];
+      }
+      float2 acc[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) acc[c] = make_float2(0.f, 0.f);
+      const int su = __builtin_amdgcn_readfirstlane(s), xd = __builtin_amdgcn_readfirstlane(xdn + z);
+      const int yd = __builtin_amdgcn_readfirstlane(ydn + z), zd = __builtin_amdgcn_readfirstlane(zdn);
+      link_terms1(Ut + ((size_t)su * 3 + 0) * 9, Ut + ((size_t)xd * 3 + 0) * 9, xu, xm, acc);
+      link_terms1(Ut + ((size_t)su * 3 + 2) * 9, Ut + ((size_t)zd * 3 + 2) * 9, zn, zm, acc);
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (wave < 3) yu[c] = rows[par][wave + 1][c][l64];
+        if (wave > 0) ym[c] = rows[par][wave - 1][c][l64];
+      }
+      link_terms1(Ut + ((size_t)su * 3 + 1) * 9, Ut + ((size_t)yd * 3 + 1) * 9, yu, ym, acc);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const unsigned ro = (unsigned)((3 * su + c) * 64);
+        const float2 ctr = zc[c];
+        float2 o;
+        o.x = cf.x * acc[c].x + cf.w * ctr.x;
+        o.y = cf.x * acc[c].y + cf.w * ctr.y;
+        if (HAS_R) {
+          const float2 rv = ld_nt(r + vo + ro + l64);
+          o.x += cf.y * rv.x;
+          o.y += cf.y * rv.y;
+        }
+        if (OLD) {
+          const float2 ov = ld_nt(old + vo + ro + l64);
+          o.x += cf.z * ov.x;
+          o.y += cf.z * ov.y;
+        } else {  // imaginary part of a complex shift
+          o.x -= cf.z * ctr.y;
+          o.y += cf.z * ctr.x;
+        }
+        st_nt(out + vo + ro + l64, o);
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        zm[c] = zc[c];
+        zc[c] = zn[c];
+      }
+    }
+  }
+}
+
 // BF16 storage of the filter vectors: a complex value is two bfloat16 in one
 // 32-bit word (real part in the low half); all arithmetic stays FP32.
 __device__ __forceinline__ float2 bf2_unpack(unsigned u) {
@@ -1088,6 +1215,34 @@ hipError_t launch_filter_step(const float2* U32, const void* in_, const void* r_
     return hipGetLastError();
   }
   if (bw != 64) return hipErrorInvalidValue;
+  if (zrun > 0) {
+    const int V3 = L * L * L;
+    if (L % zrun != 0 || W % 4 != 0) return hipErrorInvalidValue;
+    const dim3 gy((unsigned)(V3 / (4 * L)) * (unsigned)(K / 64) * (unsigned)nts);
+#define LAPH_Y64_CASE(NZ)                                                                                      \
+  case NZ:                                                                                                     \
+    if (r && old)                                                                                              \
+      k_filter_step32y<true, true, NZ><<<gy, 256, 0, st>>>(U32, in, r, old, out, coef, first_blk, L, K, nts, W); \
+    else if (r)                                                                                                \
+      k_filter_step32y<true, false, NZ><<<gy, 256, 0, st>>>(U32, in, r, old, out, coef, first_blk, L, K, nts,    \
+                                                            W);                                                \
+    else if (old)                                                                                              \
+      k_filter_step32y<false, true, NZ><<<gy, 256, 0, st>>>(U32, in, r, old, out, coef, first_blk, L, K, nts,    \
+                                                            W);                                                \
+    else                                                                                                       \
+      k_filter_step32y<false, false, NZ><<<gy, 256, 0, st>>>(U32, in, r, old, out, coef, first_blk, L, K, nts,   \
+                                                             W);                                               \
+    return hipGetLastError();
+    switch (zrun) {
+      LAPH_Y64_CASE(1)
+      LAPH_Y64_CASE(2)
+      LAPH_Y64_CASE(4)
+      LAPH_Y64_CASE(8)
+      default:
+        return hipErrorInvalidValue;
+    }
+#undef LAPH_Y64_CASE
+  }
   if (r)
     k_filter_step32<true><<<site_grid(L, K, nts), 256, 0, st>>>(U32, in, r, old, out, coef, cab, first_blk, L, K, nts,
                                                               old != nullptr, W);

@@ -199,17 +199,18 @@ def test_filter_block_32_shapes(lp, gpu, L, zrun, monkeypatch):
     b.close()
 
 
+@pytest.mark.parametrize("width", [64, 128])
 @pytest.mark.parametrize("zrun", [0, 1, 2, 4, 8])
-def test_filter_z_runs(lp, gpu, zrun, monkeypatch):
-    """The FP32 128-vector filter step in work-groups of 4 y-rows, each
-    wavefront walking zrun sites of its z-line (z-neighbour rows shifted
+def test_filter_z_runs(lp, gpu, zrun, width, monkeypatch):
+    """The FP32 64- and 128-vector filter steps in work-groups of 4 y-rows,
+    each wavefront walking zrun sites of its z-line (z-neighbour rows shifted
     through registers, y-neighbours of the inner rows through LDS; at L = 8
     the runs and the groups meet the periodic edges in z and y), and the
-    one-site-per-wavefront step (zrun 0), give the dense oracle's spectrum in
+    one-site-per-wavefront steps (zrun 0), give the dense oracle's spectrum in
     both strip orders."""
     L, nts, nev, deg, tol = 8, 2, 40, 8, 1e-10
     data = smeared(L, 57, 0.4, nts)
-    monkeypatch.setenv("LAPH_BW", "128")
+    monkeypatch.setenv("LAPH_BW", str(width))
     monkeypatch.setenv("LAPH_ZRUN", str(zrun))
     for strip in (4, 8):
         monkeypatch.setenv("LAPH_STRIP", str(strip))
