@@ -15,9 +15,10 @@
 // accumulators: with a = A_re + i A_im, b = B_re + i B_im,
 //     re += A_re B_re + A_im B_im,   im += A_im B_re - A_re B_im
 // (four MFMAs per tile and k-step).  The two 16-row x 64-column panels of an
-// iteration pass through LDS (double-buffered, rows padded by one element);
-// the next iteration's panels are fetched into registers while the current
-// ones feed the matrix cores.  Operand layout of the instruction (measured):
+// iteration pass through one LDS buffer (rows padded by one element; 33 KB,
+// so that three work-groups share a CU, as many as the 138 VGPRs allow); the
+// next iteration's panels are fetched into registers while the current ones
+// feed the matrix cores, and written to LDS between two barriers.  Operand layout of the instruction (measured):
 // lane l holds A[i = l % 16][k = l / 16] and B[k = l / 16][j = l % 16];
 // output register r of lane l is D[4 r + l / 16][l % 16].
 #include <hip/hip_runtime.h>
@@ -59,8 +60,8 @@ __device__ __forceinline__ void gram_unit(int NT, int units, int& unit, int& til
 __global__ __launch_bounds__(256) void k_gram_mfma(const double2* __restrict__ A, const double2* __restrict__ B,
                                                     double2* __restrict__ C, int n, int K, int l, int a,
                                                     int rows_chunk, int S, int nts, size_t per_ts, double alpha) {
-  __shared__ double2 pa[2][kGramRows][kGramPad];
-  __shared__ double2 pb[2][kGramRows][kGramPad];
+  __shared__ double2 pa[kGramRows][kGramPad];
+  __shared__ double2 pb[kGramRows][kGramPad];
   const int T = a / 64;
   int unit, tile, ti, tj;
   gram_unit(T * (T + 1) / 2, nts * S, unit, tile);
@@ -86,33 +87,32 @@ __global__ __launch_bounds__(256) void k_gram_mfma(const double2* __restrict__ A
     rb2 = Bb[o_ + 8 * (size_t)K];                                   \
     rb3 = Bb[o_ + 12 * (size_t)K];                                  \
   } while (0)
-#define LAPH_GRAM_STASH(buf)                                        \
+#define LAPH_GRAM_STASH()                                           \
   do {                                                              \
-    pa[buf][fr][fc] = ra0;                                          \
-    pa[buf][fr + 4][fc] = ra1;                                      \
-    pa[buf][fr + 8][fc] = ra2;                                      \
-    pa[buf][fr + 12][fc] = ra3;                                     \
-    pb[buf][fr][fc] = rb0;                                          \
-    pb[buf][fr + 4][fc] = rb1;                                      \
-    pb[buf][fr + 8][fc] = rb2;                                      \
-    pb[buf][fr + 12][fc] = rb3;                                     \
+    pa[fr][fc] = ra0;                                               \
+    pa[fr + 4][fc] = ra1;                                           \
+    pa[fr + 8][fc] = ra2;                                           \
+    pa[fr + 12][fc] = ra3;                                          \
+    pb[fr][fc] = rb0;                                               \
+    pb[fr + 4][fc] = rb1;                                           \
+    pb[fr + 8][fc] = rb2;                                           \
+    pb[fr + 12][fc] = rb3;                                          \
   } while (0)
   d4v re00 = {0, 0, 0, 0}, re01 = re00, re10 = re00, re11 = re00;
   d4v im00 = re00, im01 = re00, im10 = re00, im11 = re00;
   const int nit = rows_chunk / kGramRows;
   LAPH_GRAM_FETCH(0);
-  LAPH_GRAM_STASH(0);
+  LAPH_GRAM_STASH();
   __syncthreads();
   const int li = lane & 15, lk = lane >> 4;
   const int ci = qi * 32 + li, cj = qj * 32 + li;
   for (int it = 0; it < nit; ++it) {
-    const int buf = it & 1;
     if (it + 1 < nit) LAPH_GRAM_FETCH(it + 1);
 #pragma unroll
     for (int kk = 0; kk < kGramRows / 4; ++kk) {
       const int k = kk * 4 + lk;
-      const double2 a0 = pa[buf][k][ci], a1 = pa[buf][k][ci + 16];
-      const double2 b0 = pb[buf][k][cj], b1 = pb[buf][k][cj + 16];
+      const double2 a0 = pa[k][ci], a1 = pa[k][ci + 16];
+      const double2 b0 = pb[k][cj], b1 = pb[k][cj + 16];
       // the eight accumulators take their first product, then their second:
       // no MFMA waits on the one just issued
       re00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.x, b0.x, re00, 0, 0, 0);
@@ -132,7 +132,8 @@ __global__ __launch_bounds__(256) void k_gram_mfma(const double2* __restrict__ A
       im10 = __builtin_amdgcn_mfma_f64_16x16x4f64(-a1.x, b0.y, im10, 0, 0, 0);
       im11 = __builtin_amdgcn_mfma_f64_16x16x4f64(-a1.x, b1.y, im11, 0, 0, 0);
     }
-    if (it + 1 < nit) LAPH_GRAM_STASH(buf ^ 1);
+    __syncthreads();  // every wavefront has read the panels
+    if (it + 1 < nit) LAPH_GRAM_STASH();
     __syncthreads();
   }
 #undef LAPH_GRAM_FETCH
