@@ -189,9 +189,10 @@ typedef struct laph_batch laph_batch;
 /* Environment controls of a batch, read by laph_batch_create (for tuning and
  * tests; the defaults are the measured best):
  *   LAPH_LANES=1..4         host lanes (threads, streams) the timeslices are split over (1)
- *   LAPH_BW=64|128|256      vectors per filter block (else timed on the first solve)
+ *   LAPH_BW=32|64|128|256   vectors per filter block (else timed on the first solve; 32 only when 64
+ *                           does not divide the search space)
  *   LAPH_STRIP=w            sweep strip width of the stencils, w divides L (else timed)
- *   LAPH_ZRUN=0|1|2|4|8     FP32 128-vector filter step in work-groups of 4 y-rows (L a multiple of 4):
+ *   LAPH_ZRUN=0|1|2|4|8     FP32 64-/128-vector filter step in work-groups of 4 y-rows (L a multiple of 4):
  *                           z-sites per wavefront, dividing L (8, else 4); 0: one site per wavefront
  *   LAPH_FILTER_FORM=0|1|2  T_m(B) applied as a product over its Leja-ordered roots, the correction
  *                           pass as a product over the roots of (T_m(y) - T_m(theta)) / (y - theta)
@@ -217,7 +218,11 @@ typedef struct laph_batch laph_batch;
  *                           sums (else both timed on the first solve, with the kernel)
  *   LAPH_GRAM_MFMA=0|1      Gram products on rocBLAS zgemm (0) or on the hand-written FP64 MFMA
  *                           kernel (1); else timed on the first solve
- *   LAPH_TRACE=1            per-cycle trace of the first timeslice on stderr
+ *   LAPH_TUNE_FILE=path     layout cache: the filter and Gram layouts of a shape (device, L, K,
+ *                           timeslices) are read from this text file instead of timed on the first
+ *                           solve, and appended to it after timing (not used while LAPH_BW,
+ *                           LAPH_STRIP, LAPH_GRAM_SPLIT or LAPH_GRAM_MFMA is set)
+ *   LAPH_TRACE              (set) per-cycle trace of the first timeslice on stderr
  *   LAPH_NET_TARGET=1       laph_net_train_step fits the span of all K Ritz vectors of the last solve
  *                           (else its nev eigenvectors)
  *   LAPH_TEST_POISON=t      (tests) NaN written into one row of timeslice t's unlocked columns in
@@ -278,6 +283,8 @@ typedef struct laph_batch_stats {
                               extra batch entries, added in order), timed with gram_blocks */
   int gram_mfma;           /* 1: the Gram products ran on the hand-written FP64 MFMA kernel (the
                               64 x 64 tiles of the upper triangle), 0: on rocBLAS zgemm */
+  int layout_from_cache;   /* 1: the filter and Gram layouts were read from the LAPH_TUNE_FILE
+                              cache instead of timed on the first solve */
 } laph_batch_stats;
 
 int laph_batch_create(laph_batch** out, const laph_params* params, const laph_batch_opts* opts);

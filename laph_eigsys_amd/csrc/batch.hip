@@ -1229,18 +1229,82 @@ static int time_correction_pass(laph_batch* b, int reps, float* ms) {
   return LAPH_OK;
 }
 
-// Layout of the filter: block width (two candidates of the storage format)
-// and sweep strip width (L, or 4 or 8 when they divide L), each pair timed on
-// lane 0's timeslices with every column active (the kernels only stream the
-// buffers; their contents do not change the time); the fastest is kept.
-// LAPH_BW / LAPH_STRIP fix either.
+// Layout cache (LAPH_TUNE_FILE): the layouts the first solve chose, one text
+// line per shape and device,
+//   "<arch> cu<CUs> L<L> K<K> nt<lane 0 timeslices> s<bf16> f<prod><qform> z<zrun>
+//    : bw <width> strip <strip> split <row blocks> chunks <site chunks> mfma <0|1>"
+// read before timing (the first valid line of the key wins) and appended
+// after a miss, so that later processes on the same shape skip the timing
+// (several seconds at 32^3 x 64) and pick the same layout.  Used only while no
+// layout control (LAPH_BW, LAPH_STRIP, LAPH_GRAM_SPLIT, LAPH_GRAM_MFMA) is set.
+static std::string tune_key(const laph_batch* b) {
+  hipDeviceProp_t pr;
+  if (hipGetDeviceProperties(&pr, b->p.device) != hipSuccess) return std::string();
+  char k[320];
+  std::snprintf(k, sizeof k, "%s cu%d L%d K%d nt%d s%d f%d%d z%d", pr.gcnArchName, pr.multiProcessorCount, b->p.L,
+                b->K, b->lanes[0].nt, (int)b->bf16, (int)b->prod, (int)b->qform, b->zrun);
+  return std::string(k);
+}
+
+static bool tune_lookup(laph_batch* b, const char* path, const std::string& key) {
+  FILE* f = std::fopen(path, "r");
+  if (!f) return false;
+  char line[512];
+  bool hit = false;
+  const std::vector<int> cand = block_widths(b);
+  while (!hit && std::fgets(line, sizeof line, f)) {
+    const char* sep = std::strstr(line, " : ");
+    if (!sep || (size_t)(sep - line) != key.size() || std::strncmp(line, key.c_str(), key.size()) != 0) continue;
+    int bw = 0, strip = 0, split = 0, ks = 0, mf = 0;
+    if (std::sscanf(sep + 3, "bw %d strip %d split %d chunks %d mfma %d", &bw, &strip, &split, &ks, &mf) != 5)
+      continue;
+    const bool ok = std::find(cand.begin(), cand.end(), bw) != cand.end() && b->K % bw == 0 && strip >= 1 &&
+                    b->p.L % strip == 0 && split >= 1 && split <= 8 && ks >= 1 && ks <= b->gram_ksplit_max &&
+                    b->n % ks == 0 && (mf == 0 || (mf == 1 && split == 1 && gram_mfma_fits(b->n, b->K, ks)));
+    if (!ok) continue;
+    b->bw = bw;
+    b->strip = strip;
+    b->gram_split = split;
+    b->gram_ksplit = ks;
+    b->gram_mfma = mf;
+    hit = true;
+  }
+  std::fclose(f);
+  return hit;
+}
+
+static void tune_store(const laph_batch* b, const char* path, const std::string& key) {
+  FILE* f = std::fopen(path, "a");
+  if (!f) return;  // an unwritable cache only costs the next process the timing
+  std::fprintf(f, "%s : bw %d strip %d split %d chunks %d mfma %d\n", key.c_str(), b->bw, b->strip, b->gram_split,
+               b->gram_ksplit, b->gram_mfma);
+  std::fclose(f);
+}
+
+// Layout of the filter: block width (the candidates of the storage format;
+// 32-vector blocks only where 64 does not divide K: elsewhere they were never
+// faster, 14.9 against 9.1 and 6.0 ms per step at 32^3) and sweep strip width
+// (L, or 4 or 8 when they divide L), each pair timed on lane 0's timeslices
+// with every column active (the kernels only stream the buffers; their
+// contents do not change the time); the fastest is kept.  LAPH_BW /
+// LAPH_STRIP fix either.  Then the Gram layout.  LAPH_TUNE_FILE caches both.
 static int autotune_block_width(laph_batch* b) {
   laph_lane* ln = &b->lanes[0];
+  const char* tf = std::getenv("LAPH_TUNE_FILE");
+  const bool cache = tf && *tf && b->bw == 0 && !b->strip_fixed && !b->gram_split_fixed;
+  const std::string key = cache ? tune_key(b) : std::string();
+  if (!key.empty() && tune_lookup(b, tf, key)) {
+    b->tuned = b->tuned_from_cache = true;
+    if (b->trace)
+      std::fprintf(stderr, "[laph] layout from %s: block width %d, strip %d, Gram %s, row blocks %d, site chunks %d\n",
+                   tf, b->bw, b->strip, b->gram_mfma ? "MFMA kernel" : "rocBLAS", b->gram_split, b->gram_ksplit);
+    return LAPH_OK;
+  }
   ln->m = 0;  // layouts are timed at the batch's degree
   LAPH_HIP(hipMemsetAsync(b->first_blk + ln->t0, 0, ln->nt * sizeof(int), ln->st));
   std::vector<int> widths, strips;
   for (int w : block_widths(b))
-    if (b->K % w == 0 && (b->bw == 0 || b->bw == w)) widths.push_back(w);
+    if (b->K % w == 0 && (b->bw == w || (b->bw == 0 && (w != 32 || b->bf16 || b->K % 64 != 0)))) widths.push_back(w);
   if (b->strip_fixed) {
     strips.push_back(b->strip);
   } else {
@@ -1301,6 +1365,7 @@ static int autotune_block_width(laph_batch* b) {
   }
   LAPH_HIP(hipGraphExecDestroy(ln->graph));  // captured for the last layout timed
   ln->graph = nullptr;
+  if (!key.empty()) tune_store(b, tf, key);
   if (b->trace)
     std::fprintf(stderr, "[laph] filter block width %d, strip %d, Gram %s, row blocks %d, site chunks %d\n", wbest,
                  sbest, b->gram_mfma ? "MFMA kernel" : "rocBLAS", b->gram_split, b->gram_ksplit);
@@ -1368,6 +1433,7 @@ static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
   s.gram_blocks = b->gram_split;
   s.gram_chunks = b->gram_ksplit;
   s.gram_mfma = b->gram_mfma;
+  s.layout_from_cache = b->tuned_from_cache;
   s.filter_strip = b->strip;
   s.filter_zrun = b->bf16 == 0 && (b->bw == 128 || b->bw == 64) ? laph::stencil_zrun(b) : 0;
   s.start_window = b->window0;

@@ -105,6 +105,7 @@ struct laph_batch {
   bool strip_fixed = false;  // LAPH_STRIP given
   int zrun = 0;  // z-sites per wavefront of the y-row work-group filter step (LAPH_ZRUN; 0: one site each)
   bool tuned = false;        // filter layout chosen (first solve)
+  bool tuned_from_cache = false;  // ... read from the LAPH_TUNE_FILE layout cache instead of timed
   int poison_ts = -1;  // LAPH_TEST_POISON: NaN into one row of this timeslice's active block at cycle 2 (tests)
   int qr_cycles = 0;  // cycles (and the start block) orthonormalised by explicit CholQR2 (LAPH_QR_CYCLES)
   // laph_batch_write: phase-fixed copy in the file layout, V^H V, rocBLAS handle on st,

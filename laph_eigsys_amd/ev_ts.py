@@ -3,6 +3,7 @@
 
     python -m laph_eigsys_amd.ev_ts --conf 1000 [--phase_reset 1] [--params parameters.txt]
         [--solver batched|plain] [--tstart T0 --tend T1] [--group G] [--mem-gb M]
+        [--tune-file PATH]
 
 Reads <config_path>/conf.%04d (ILDG LIME, io.cpp:22-146), HYP-smears every
 timeslice (alpha_1, alpha_2, iter), computes the NEV eigenpairs of the
@@ -21,7 +22,10 @@ Solvers:
   batched  the mixed-precision batched solver over groups of timeslices (the
            damped interval adapts per timeslice, LAM_C is not used); the
            group size fits the work buffers into --mem-gb of HBM, so 48^3 x 96
-           streams through in groups of 24.
+           streams through in groups of 24.  The first solve of a process
+           times the filter and Gram layouts (several seconds at 32^3 x 64);
+           --tune-file keeps them in a small text file (LAPH_TUNE_FILE) so
+           that the next configurations of a production run read them.
 
 The reference divides LT over MPI ranks (ev_ts.cpp:96-126); one process
 here takes [tstart, tend] (default: all of LT), the share one rank would get.
@@ -237,7 +241,11 @@ def main(argv=None):
     ap.add_argument("--tol", type=float, default=1e-10, help="residual |D v - lambda v| (batched), EPS tol (plain)")
     ap.add_argument("--device", type=int, default=0)
     ap.add_argument("--json", default=None, help="write the timing summary here")
+    ap.add_argument("--tune-file", default=None,
+                    help="layout cache of the batched solver (read, and appended after timing a new shape)")
     args = ap.parse_args(argv)
+    if args.tune_file:
+        os.environ["LAPH_TUNE_FILE"] = args.tune_file
     pars = IO().set_values(args.params)
     pars.print_summary()
     print(f"calculating config {args.conf}")

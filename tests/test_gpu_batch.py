@@ -406,3 +406,42 @@ def test_gram_row_blocks_and_site_chunks(lp, gpu, split, mfma, monkeypatch):
         ev, V, ph, res = b.get(t)
         check_pairs(ev, V, res, N.dense_D(U.reshape(L, L, L, 3, 3, 3)), nev, tol)
     b.close()
+
+
+def test_layout_cache(lp, gpu, tmp_path, monkeypatch):
+    """LAPH_TUNE_FILE: the first solve times the layouts and appends them
+    under the shape's key; a later batch of the same shape reads them instead
+    of timing (layout_from_cache), skipping lines whose values are not a
+    valid layout, and solves to the same eigenvalues bit for bit; a batch of
+    another shape misses and appends its own line."""
+    L, nts, nev, deg, tol = 8, 2, 40, 8, 1e-10
+    data = smeared(L, 23, 0.4, nts)
+    cache = tmp_path / "layouts.txt"
+    monkeypatch.setenv("LAPH_TUNE_FILE", str(cache))
+
+    def run(nsub=128):
+        b = lp.Batch(L, deg, 12.0, 0.0, nts, nev, nsub=nsub, tol=tol, seed=3, device=gpu)
+        for t, (ts, _) in enumerate(data):
+            b.set_timeslice(t, ts, 0.62, 0.62, 3)
+        st = b.solve()
+        assert st.converged == nts
+        ev = [b.get(t, want_vectors=False)[0] for t in range(nts)]
+        b.close()
+        return st, ev
+
+    st1, ev1 = run()
+    assert st1.layout_from_cache == 0
+    lines = cache.read_text().splitlines()
+    assert len(lines) == 1 and " : bw " in lines[0]
+    key, val = lines[0].split(" : ")
+    # an invalid entry for the same key (a width that does not divide K) comes first and is skipped
+    cache.write_text(f"{key} : bw 48 strip 3 split 1 chunks 1 mfma 0\n{lines[0]}\n")
+    st2, ev2 = run()
+    assert st2.layout_from_cache == 1
+    for f in ("filter_block", "filter_strip", "gram_blocks", "gram_chunks", "gram_mfma"):
+        assert getattr(st2, f) == getattr(st1, f), f
+    for a, b in zip(ev1, ev2):
+        assert np.array_equal(a, b)
+    st3, _ = run(nsub=192)  # another search space: a miss, timed and appended
+    assert st3.layout_from_cache == 0
+    assert len(cache.read_text().splitlines()) == 3
