@@ -171,6 +171,7 @@ def main():
     # dominant kernel: FP32 filter stencil, timed alone on the device
     ms_launch, bytes_launch = b.time_filter(reps=args.filter_reps)
     gram = gram_rate(b, st)
+    eps_res = eps_residual(lp, b, cfg, vol, sorted({0, nts - 1}), plain_lambda_c(L, NEV))
     b.close()
     achieved = bytes_launch / (ms_launch * 1e-3)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
@@ -230,13 +231,33 @@ def main():
                        "seconds_per_timeslice": spread(plain_times),
                        "max_rel_eigenvalue_difference_vs_batched": plain_dev},
         "speedup_vs_plain_fp64": (value / plain_value) if plain_value else None,
-        "max_residual": tot["maxres"], "cycles": tot["cycles"],
+        "max_residual": tot["maxres"], "eps_relative_residual": eps_res, "cycles": tot["cycles"],
         "fallbacks": {"recoveries": tot["recoveries"], "rr_cholqr": tot["rr_fallbacks"],
                       "shifted_cholqr": tot["shifted"]},
         "upload_map_hyp_seconds": t_hyp,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def eps_residual(lp, b, cfg, vol, tslices, lam_c):
+    """The residual that the reference's -eps_tol bounds, ||T_DEG(B) v - theta v|| / |theta|
+    with theta = T_DEG(c lambda + a) (ev_ts.cpp:263-292), for the batched solver's
+    phase-fixed eigenpairs (v, lambda) of the given timeslices, B taken at the
+    plain drop-in's lambda_c; T_DEG(B) v through the MatShell entry point."""
+    c = 2.0 / (LAMBDA_L - lam_c)
+    a = 1.0 + c * lam_c
+    worst = 0.0
+    for t in tslices:
+        ev, V, _, _ = b.get(t)
+        theta = np.cos(DEG * np.arccos((c * ev + a).astype(np.complex128))).real
+        with lp.Tslice(L, DEG, LAMBDA_L, lam_c, ts_config=cfg[t * vol:(t + 1) * vol]) as ts:
+            ts.smearing_hyp(0.62, 0.62, 3)
+            for j in range(V.shape[0]):
+                y = ts.MatMult_Laplacian2D(V[j])
+                worst = max(worst, float(np.linalg.norm(y - theta[j] * V[j]) / abs(theta[j])))
+    return {"max": worst, "timeslices": list(tslices), "lambda_c": lam_c,
+            "definition": f"||T_{DEG}(B) v - theta v|| / |theta| over the N_ev phase-fixed pairs"}
 
 
 def profile_filter(lp, args):
