@@ -189,8 +189,7 @@ typedef struct laph_batch laph_batch;
 /* Environment controls of a batch, read by laph_batch_create (for tuning and
  * tests; the defaults are the measured best):
  *   LAPH_LANES=1..4         host lanes (threads, streams) the timeslices are split over (1)
- *   LAPH_BW=32|64|128|256   vectors per filter block (else timed on the first solve; 32 only when 64
- *                           does not divide the search space)
+ *   LAPH_BW=32|64|128|256   vectors per filter block (else timed on the first solve)
  *   LAPH_STRIP=w            sweep strip width of the stencils, w divides L (else timed)
  *   LAPH_ZRUN=0|1|2|4|8     FP32 64-/128-vector filter step in work-groups of 4 y-rows (L a multiple of 4):
  *                           z-sites per wavefront, dividing L (8, else 4); 0: one site per wavefront

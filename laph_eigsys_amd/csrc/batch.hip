@@ -1214,13 +1214,24 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
 
 // Device time of `reps` replays of lane 0's correction pass at the current
 // block width (graph recaptured for that width).
-static int time_correction_pass(laph_batch* b, int reps, float* ms) {
+// reject_above > 0: a layout whose first (warm-up) pass already takes longer
+// than that many ms is reported at reps x that time, without further passes.
+static int time_correction_pass(laph_batch* b, int reps, float* ms, float reject_above = 0.f) {
   laph_lane* ln = &b->lanes[0];
   const View v{b, ln};
   if (ln->graph) LAPH_HIP(hipGraphExecDestroy(ln->graph));
   ln->graph = nullptr;
   LAPH_TRY(capture(v, &ln->graph, [&] { return enqueue_filter(v); }));
+  LAPH_HIP(hipEventRecord(ln->ev[0], ln->st));
   LAPH_HIP(hipGraphLaunch(ln->graph, ln->st));  // warm-up
+  LAPH_HIP(hipEventRecord(ln->ev[1], ln->st));
+  LAPH_HIP(hipEventSynchronize(ln->ev[1]));
+  float first = 0.f;
+  LAPH_HIP(hipEventElapsedTime(&first, ln->ev[0], ln->ev[1]));
+  if (reject_above > 0.f && first > reject_above) {
+    *ms = first * reps;
+    return LAPH_OK;
+  }
   LAPH_HIP(hipEventRecord(ln->ev[0], ln->st));
   for (int r = 0; r < reps; ++r) LAPH_HIP(hipGraphLaunch(ln->graph, ln->st));
   LAPH_HIP(hipEventRecord(ln->ev[1], ln->st));
@@ -1281,13 +1292,14 @@ static void tune_store(const laph_batch* b, const char* path, const std::string&
   std::fclose(f);
 }
 
-// Layout of the filter: block width (the candidates of the storage format;
-// 32-vector blocks only where 64 does not divide K: elsewhere they were never
-// faster, 14.9 against 9.1 and 6.0 ms per step at 32^3) and sweep strip width
-// (L, or 4 or 8 when they divide L), each pair timed on lane 0's timeslices
-// with every column active (the kernels only stream the buffers; their
-// contents do not change the time); the fastest is kept.  LAPH_BW /
-// LAPH_STRIP fix either.  Then the Gram layout.  LAPH_TUNE_FILE caches both.
+// Layout of the filter: block width (the candidates of the storage format,
+// widest first) and sweep strip width (L, or 4 or 8 when they divide L), each
+// pair timed on lane 0's timeslices with every column active (the kernels
+// only stream the buffers; their contents do not change the time); the
+// fastest is kept.  A pair whose warm-up pass already takes 1.5 x the best
+// pass so far is not timed further (the 32-vector blocks at 32^3: 14.9
+// against 6.0 ms per step).  LAPH_BW / LAPH_STRIP fix either.  Then the Gram
+// layout.  LAPH_TUNE_FILE caches both.
 static int autotune_block_width(laph_batch* b) {
   laph_lane* ln = &b->lanes[0];
   const char* tf = std::getenv("LAPH_TUNE_FILE");
@@ -1304,7 +1316,8 @@ static int autotune_block_width(laph_batch* b) {
   LAPH_HIP(hipMemsetAsync(b->first_blk + ln->t0, 0, ln->nt * sizeof(int), ln->st));
   std::vector<int> widths, strips;
   for (int w : block_widths(b))
-    if (b->K % w == 0 && (b->bw == w || (b->bw == 0 && (w != 32 || b->bf16 || b->K % 64 != 0)))) widths.push_back(w);
+    if (b->K % w == 0 && (b->bw == 0 || b->bw == w)) widths.push_back(w);
+  std::sort(widths.rbegin(), widths.rend());
   if (b->strip_fixed) {
     strips.push_back(b->strip);
   } else {
@@ -1320,7 +1333,7 @@ static int autotune_block_width(laph_batch* b) {
       b->bw = w;
       b->strip = sw;
       float ms = 0;
-      LAPH_TRY(time_correction_pass(b, 2, &ms));
+      LAPH_TRY(time_correction_pass(b, 2, &ms, first ? 0.f : 0.75f * best));
       if (first || ms < best) {
         best = ms;
         wbest = w;
@@ -1341,7 +1354,7 @@ static int autotune_block_width(laph_batch* b) {
     float best_ms = 0;
     int best_split = 1, best_ks = 1, best_mfma = 0;
     bool first_g = true;
-    for (int mf : {0, 1})
+    for (int mf : {1, 0})
       for (int split : {1, 2})
       for (int ks : {1, 2, 4, 8}) {
         if (b->n % ks != 0 || ks > b->gram_ksplit_max) continue;
@@ -1349,12 +1362,18 @@ static int autotune_block_width(laph_batch* b) {
         b->gram_mfma = mf;
         b->gram_split = split;
         b->gram_ksplit = ks;
-        LAPH_TRY(gram(v, q, v.X(), v.X(), b->G2 + v.kk(), 1.0));  // warm-up
+        // warm-up, timed: a layout already 1.5 x slower than the best is not timed further
+        LAPH_HIP(hipEventRecord(ln->ev[0], ln->st));
+        LAPH_TRY(gram(v, q, v.X(), v.X(), b->G2 + v.kk(), 1.0));
+        LAPH_HIP(hipEventRecord(ln->ev[1], ln->st));
+        LAPH_HIP(hipEventSynchronize(ln->ev[1]));
+        float ms = 0;
+        LAPH_HIP(hipEventElapsedTime(&ms, ln->ev[0], ln->ev[1]));
+        if (!first_g && ms > 0.75f * best_ms) continue;
         LAPH_HIP(hipEventRecord(ln->ev[0], ln->st));
         for (int r = 0; r < 2; ++r) LAPH_TRY(gram(v, q, v.X(), v.X(), b->G2 + v.kk(), 1.0));
         LAPH_HIP(hipEventRecord(ln->ev[1], ln->st));
         LAPH_HIP(hipEventSynchronize(ln->ev[1]));
-        float ms = 0;
         LAPH_HIP(hipEventElapsedTime(&ms, ln->ev[0], ln->ev[1]));
         if (first_g || ms < best_ms) best_ms = ms, best_split = split, best_ks = ks, best_mfma = mf;
         first_g = false;
