@@ -1195,6 +1195,18 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
                    "shifted %d\n", it, m, passes, lc[0], mu[b->nev - 1], mu[K - 1], std::sqrt(rs[0]),
                    std::sqrt(rs[b->nev - 1]), worst, b->nlock[0], ln->lock, ln->min_pivot2, ln->min_pivot_ts,
                    ln->rr_fallbacks, ln->shifted);
+      // per 64-column block of timeslice 0: largest residual of its unconverged wanted columns, and
+      // acosh(theta) (theta in the next cycle's window) of its slowest wanted column
+      const double c = 2.0 / (b->p.lambda_l - lc[0]);
+      std::fprintf(stderr, "[laph]   blocks:");
+      for (int blk = 0; blk * 64 < b->nev; ++blk) {
+        double rmax = 0.0;
+        const int jend = std::min((blk + 1) * 64, b->nev);
+        for (int j = std::max(blk * 64, b->nlock[0]); j < jend; ++j) rmax = std::max(rmax, std::sqrt(rs[j]));
+        const double th = 1.0 + c * (lc[0] - mu[jend - 1]);
+        std::fprintf(stderr, "  [%d] res %.2e psi %.4f", blk, rmax, th > 1.0 ? std::acosh(th) : 0.0);
+      }
+      std::fprintf(stderr, "\n");
     }
     float a = 0, c = 0;
     LAPH_HIP(hipEventElapsedTime(&a, ln->ev[0], ln->ev[1]));
