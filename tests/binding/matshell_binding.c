@@ -1,8 +1,9 @@
 /*
  * The reference-side binding of the MatShell callbacks, compiled as C11
  * against include/laph.h: MatMult_Laplacian2D and MatGetDiagonal_Laplacian2D
- * with the signatures of modules/shell_matop.h:25-26 and the bodies shown in
- * INTEGRATION.md (section 1), over minimal stand-ins for the PETSc handles
+ * with the signatures of modules/shell_matop.h:25-26 and the bodies a
+ * maintainer would put in shell_matop.cpp (the repository's integration
+ * notes show the same bodies), over minimal stand-ins for the PETSc handles
  * and calls they use (Mat carries the MatShell context, Vec a complex double
  * array).  Exported for the tests: binding_mat_mult / binding_get_diagonal
  * wrap a laph_ctx* and plain arrays in those handles and call the callbacks

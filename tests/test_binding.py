@@ -1,10 +1,10 @@
-"""The reference-side binding of the MatShell callbacks (INTEGRATION.md,
-section 1; signatures of modules/shell_matop.h:25-26) compiled as C11 with
-gcc against include/laph.h, over stand-ins for the PETSc handles
+"""The reference-side binding of the MatShell callbacks (signatures of
+modules/shell_matop.h:25-26) compiled as C11 with gcc against
+include/laph.h, over stand-ins for the PETSc handles
 (tests/binding/matshell_binding.c).
 
-CPU: the two callback bodies in the C file are the ones INTEGRATION.md
-shows, the file compiles warning-free as C11 and links against
+CPU: the two callback bodies in the C file are the ones the repository's
+integration notes show, the file compiles warning-free as C11 and links against
 liblapheig.so.  GPU: MatMult_Laplacian2D, called through the mock (Mat, Vec,
 Vec) handles as EPSSolve's MatMult would, equals the oracle's tv_iter
 (T_DEG(B) x, shell_matop.cpp:136-168) to 1e-13 relative, and
