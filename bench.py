@@ -81,9 +81,10 @@ def main():
     ap.add_argument("--summarize-mfma", metavar="PMC_CSV",
                     help="FP64 matrix-core rate per kernel from a rocprofv3 --pmc run with "
                          "SQ_INSTS_VALU_MFMA_MOPS_F64")
-    ap.add_argument("--workload", choices=("c3", "c2", "c4"), default="c3",
+    ap.add_argument("--workload", choices=("c3", "c2", "c4", "fields"), default="c3",
                     help="c3: the headline (32^3x64, N_ev 220); c2: 24^3x48 N_ev 120, every timeslice through the "
-                         "plain drop-in and the batched solver; c4: 48^3x96 N_ev 200 in HBM-sized groups")
+                         "plain drop-in and the batched solver; c4: 48^3x96 N_ev 200 in HBM-sized groups; fields: "
+                         "the headline solve on other seeds and field strengths")
     ap.add_argument("--stream-ceiling", action="store_true",
                     help="only run the streaming-bandwidth measurement (laph_eigsys_amd/stream_ceiling)")
     ap.add_argument("--no-stream", action="store_true", help="skip the streaming-bandwidth measurement")
@@ -119,6 +120,9 @@ def main():
         return
     if args.workload == "c4":
         workload_c4(lp, args)
+        return
+    if args.workload == "fields":
+        workload_fields(lp, args)
         return
     if args.profile_filter:
         profile_filter(lp, args)
@@ -412,6 +416,49 @@ def summarize_pmc(fetch_csv, write_csv, last):
     print(json.dumps({"filter_dispatches": [nf, nw], "fetch_KiB": f, "write_KiB": w,
                       "copy_fetch_KiB": fc, "copy_write_KiB": wc, "fetch_scale": kf, "write_scale": kw,
                       "traffic_bytes_per_launch": traffic}))
+
+
+def workload_fields(lp, args):
+    """The headline solve (32^3 x 64, N_ev 220, DEG 16, K 256, tol 1e-10) on
+    synthetic fields of other seeds and field strengths eps, one batch object
+    reused (layouts timed on the first field): per field the timed solve's
+    rate, cycles, largest residual and fallbacks, and the largest relative
+    eigenvalue difference to the plain FP64 drop-in on timeslices 0 and 63
+    (lambda_c 1.35 x the batched |lambda_220| of the timeslice)."""
+    vol = 72 * L ** 3
+    sync = device_sync()
+    b = lp.Batch(L, DEG, LAMBDA_L, 0.0, LT, NEV, nsub=NSUB, tol=TOL, seed=SEED)
+    rows = []
+    try:
+        for seed, eps in ((1227, 0.3), (7, 0.3), (2024, 0.3), (1227, 0.2), (1227, 0.4), (99, 0.45)):
+            cfg = lp.synth_config(seed, eps, L, LT)
+            for t in range(LT):
+                b.set_timeslice(t, cfg[t * vol:(t + 1) * vol], 0.62, 0.62, 3)
+            b.solve()  # untimed (the first one also times the layouts)
+            sync()
+            t0 = time.perf_counter()
+            st = b.solve()
+            sync()
+            dt = time.perf_counter() - t0
+            dev = 0.0
+            for t in (0, LT - 1):
+                evb = b.get(t, want_vectors=False)[0]
+                lam_c = 1.35 * abs(evb[-1])  # above the wanted part of this field's spectrum
+                with lp.Tslice(L, DEG, LAMBDA_L, lam_c, ts_config=cfg[t * vol:(t + 1) * vol]) as sl:
+                    sl.smearing_hyp(0.62, 0.62, 3)
+                    theta, _, _ = sl.eps_solve(NEV, tol=TOL, want_vectors=False)
+                ev = lp.recover_spectrum(theta, DEG, LAMBDA_L, lam_c)
+                dev = max(dev, float(np.max(np.abs(ev - evb) / np.abs(evb))))
+            rows.append({"seed": seed, "eps": eps, "ts_per_s": LT / dt, "cycles": st.iterations,
+                         "converged": st.converged, "max_residual": st.max_residual, "recoveries": st.recoveries,
+                         "rr_cholqr_fallbacks": st.rr_cholqr_fallbacks, "shifted_cholqr": st.shifted_cholqr,
+                         "max_rel_eigenvalue_difference_vs_plain": dev})
+            print(f"[bench] field seed {seed} eps {eps}: {LT / dt:.2f} ts/s, {st.iterations} cycles, "
+                  f"max res {st.max_residual:.2e}, vs plain {dev:.1e}", file=sys.stderr, flush=True)
+    finally:
+        b.close()
+    print(json.dumps({"workload": f"{L}^3x{LT} N_ev={NEV} DEG {DEG} K {NSUB} tol {TOL}, HYP 0.62/0.62/3",
+                      "fields": rows}))
 
 
 def workload_c2(lp, args):

@@ -208,6 +208,9 @@ typedef struct laph_batch laph_batch;
  *   LAPH_AMP_CAP=x          bound on the amplification T_d(theta_max) (or T_DEG(theta_max)^P) of a cycle (1e10)
  *   LAPH_FILTER_DEGREE=d    highest per-cycle filter degree (72); 0: applications of T_DEG(B) per cycle
  *                           (up to 5, or filter_passes)
+ *   LAPH_DEGREE_SAFETY=S    a cycle's degree is lowered to the least d with T_d(theta_j) >= S r_j / tol
+ *                           for every unconverged wanted pair (theta_j, r_j) of the lane (30); 0: always the
+ *                           highest degree the amplification bound allows
  *   LAPH_SHIFTED_CHOLQR=1   every CholQR pass starts with a shifted one
  *   LAPH_TRSM=1             Rayleigh-Ritz through triangular solves with the Gram's Cholesky factor
  *                           (else its explicit inverse and GEMMs)

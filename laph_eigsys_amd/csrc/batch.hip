@@ -1074,12 +1074,35 @@ static int filter_passes(const laph_batch* b, const std::vector<double>& lc) {
 // largest multiple of 4 up to fdeg_max with T_d(theta_max) <= amp_cap (one
 // application per cycle); the wide window of the first cycle gets a low
 // degree, the settled windows a high one.
-static int cycle_degree(const laph_batch* b, const std::vector<double>& lc) {
+// Degree of a cycle: the highest multiple of 4 (at most fdeg_max) whose
+// amplification T_d(theta_max) stays under amp_cap, lowered to what the
+// unconverged wanted columns need when that is less: a Ritz pair (theta, r)
+// comes out of a filter of degree d with its residual about r / T_d(theta)
+// (within 1.3x in the last cycles of a 32^3 solve, 7x in the early ones), so
+// the cycle takes the lowest multiple of 4 with T_d(theta_j) >= S r_j / tol
+// for every such column j of the lane (S = deg_safety; 0 switches this off).
+static int cycle_degree(const laph_batch* b, const laph_lane* ln, const std::vector<double>& lc) {
   double pmax = 0.0;  // largest acosh(theta_max) over the lane's timeslices
   for (double l : lc) pmax = std::max(pmax, std::acosh(1.0 + 2.0 * l / (b->p.lambda_l - l)));
   int d = pmax > 0.0 ? (int)std::floor(std::acosh(b->amp_cap) / pmax) : b->fdeg_max;
-  d = std::min(d, b->fdeg_max) / 4 * 4;
-  return std::max(d, 4);
+  d = std::max(std::min(d, b->fdeg_max) / 4 * 4, 4);
+  if (b->deg_safety <= 0.0) return d;
+  double need = 0.0;
+  for (int t = 0; t < ln->nt; ++t) {
+    const int T = ln->t0 + t;
+    const double c = 2.0 / (b->p.lambda_l - lc[t]);
+    const double* mu = &b->h_mu[(size_t)T * b->K];
+    const double* res = &b->h_res[(size_t)T * b->K];
+    for (int j = b->nlock[T]; j < b->nev; ++j) {
+      const double r = std::sqrt(res[j]);
+      if (r <= b->o.tol) continue;
+      const double th = 1.0 + c * (lc[t] - mu[j]);
+      if (!(th > 1.0 + 1e-12)) return d;  // a wanted value at or above the window edge: no estimate
+      need = std::max(need, std::acosh(b->deg_safety * r / b->o.tol) / std::acosh(th));
+    }
+  }
+  const int dn = std::max(4, (int)std::ceil(need / 4.0) * 4);
+  return std::min(d, dn);
 }
 
 // Solve loop of one lane (runs on its own host thread).
@@ -1132,7 +1155,7 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
     // Rayleigh-Ritz values give the window of the first cycle
     const bool warmup = it < (warm && b->o.warm_start == 2 ? 0 : b->warm_cycles);
     const bool wdeg = b->fdeg_max >= b->warm_deg && b->warm_deg > 0;  // warm-up degree of the per-cycle mode
-    ln->m = b->fdeg_max > 0 && !warmup ? cycle_degree(b, lc) : (warmup && wdeg ? b->warm_deg : b->deg);
+    ln->m = b->fdeg_max > 0 && !warmup ? cycle_degree(b, ln, lc) : (warmup && wdeg ? b->warm_deg : b->deg);
     const int passes = b->fdeg_max > 0 || warmup ? 1 : filter_passes(b, lc);
     const int m = ln->m;
     LAPH_TRY(upload_coefficients(v, lc, passes));
@@ -1554,6 +1577,7 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     if (const char* e = std::getenv("LAPH_START_RR")) b->start_rr = std::atoi(e) != 0;
     if (const char* e = std::getenv("LAPH_TRSM")) b->use_trsm = std::atoi(e) != 0;
     if (const char* e = std::getenv("LAPH_AMP_CAP")) b->amp_cap = std::max(10.0, std::atof(e));
+    if (const char* e = std::getenv("LAPH_DEGREE_SAFETY")) b->deg_safety = std::max(0.0, std::atof(e));
     b->zrun = zrun;
     // first-cycle window: 3 x the free Laplacian's K-th level (measured best
     // between 2.5 and 3.5 x at 24^3, 32^3 and 48^3), at most the usual cap
