@@ -254,18 +254,23 @@ def test_filter_passes(lp, gpu, form, passes, monkeypatch):
     b.close()
 
 
-@pytest.mark.parametrize("fdeg,wdeg", [(24, 0), (72, 0), (0, 0), (72, 24)])
-def test_filter_degree_per_cycle(lp, gpu, fdeg, wdeg, monkeypatch):
+@pytest.mark.parametrize("fdeg,wdeg,safety", [(24, 0, None), (72, 0, None), (0, 0, None), (72, 24, None),
+                                               (72, 0, "0"), (72, 0, "3")])
+def test_filter_degree_per_cycle(lp, gpu, fdeg, wdeg, safety, monkeypatch):
     """Automatic passes (filter_passes = 0): after the warm-up cycles one
     filter per cycle of the highest degree (a multiple of 4 up to
-    LAPH_FILTER_DEGREE) whose amplification stays under the cap; 0 selects
-    T_DEG(B) applied up to 5 times per cycle; LAPH_WARM_DEG sets the degree of
-    the warm-up cycle.  All give the dense oracle's spectrum."""
+    LAPH_FILTER_DEGREE) whose amplification stays under the cap, lowered to
+    what the unconverged pairs need (margin LAPH_DEGREE_SAFETY: 30, 0 = never
+    lowered, 3 = a thin margin that may cost cycles but not accuracy); 0
+    selects T_DEG(B) applied up to 5 times per cycle; LAPH_WARM_DEG sets the
+    degree of the warm-up cycle.  All give the dense oracle's spectrum."""
     L, nts, nev, deg, tol = 8, 2, 40, 8, 1e-10
     data = smeared(L, 313, 0.4, nts)
     monkeypatch.setenv("LAPH_FILTER_DEGREE", str(fdeg))
     if wdeg:
         monkeypatch.setenv("LAPH_WARM_DEG", str(wdeg))
+    if safety is not None:
+        monkeypatch.setenv("LAPH_DEGREE_SAFETY", safety)
     b = lp.Batch(L, deg, 12.0, 0.0, nts, nev, nsub=64, tol=tol, seed=7, device=gpu)
     for t, (ts, _) in enumerate(data):
         b.set_timeslice(t, ts, 0.62, 0.62, 3)
