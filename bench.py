@@ -88,6 +88,9 @@ def main():
     ap.add_argument("--stream-ceiling", action="store_true",
                     help="only run the streaming-bandwidth measurement (laph_eigsys_amd/stream_ceiling)")
     ap.add_argument("--no-stream", action="store_true", help="skip the streaming-bandwidth measurement")
+    ap.add_argument("--filter-bound", action="store_true",
+                    help="only run laph_eigsys_amd/filter_bound: the filter step beside variants with a part taken "
+                         "away and copies with its load pattern (32^3, 64 timeslices, 256 vectors)")
     ap.add_argument("--last", type=int, default=0,
                     help="with --summarize-*: the isolated filter replays are the last LAST filter dispatches")
     args = ap.parse_args()
@@ -102,6 +105,9 @@ def main():
         return
     if args.stream_ceiling:
         print(json.dumps(stream_ceiling()))
+        return
+    if args.filter_bound:
+        print(json.dumps(run_measure("filter_bound")))
         return
     if args.cpu_worker:
         cpu_worker(args.cpu_worker)
@@ -591,17 +597,23 @@ def workload_c4(lp, args):
                       "filter_GBps": by / ms / 1e6, "filter_us_per_launch": ms * 1e3, "gram": gram}))
 
 
-def stream_ceiling():
-    """JSON of laph_eigsys_amd/stream_ceiling (built with the library): best
-    GB/s of float4 streaming kernels for 1-3 reads + 1 write and read-only."""
+def run_measure(name, *argv):
+    """JSON printed by one of the measurement programs built with the library
+    (laph_eigsys_amd/<name>, from laph_eigsys_amd/csrc/measure/<name>.hip)."""
     import subprocess
-    exe = os.path.join(ROOT, "laph_eigsys_amd", "stream_ceiling")
+    exe = os.path.join(ROOT, "laph_eigsys_amd", name)
     if not os.path.exists(exe):
         raise RuntimeError(f"{exe} missing: run __graft_entry__.build()")
-    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    r = subprocess.run([exe, *map(str, argv)], capture_output=True, text=True, timeout=300)
     if r.returncode != 0:
-        raise RuntimeError(f"stream_ceiling failed ({r.returncode}): {r.stderr.strip()}")
+        raise RuntimeError(f"{name} failed ({r.returncode}): {r.stderr.strip()}")
     return json.loads(r.stdout)
+
+
+def stream_ceiling():
+    """JSON of laph_eigsys_amd/stream_ceiling: best GB/s of float4 streaming
+    kernels for 1-3 reads + 1 write and read-only."""
+    return run_measure("stream_ceiling")
 
 
 def device_sync():

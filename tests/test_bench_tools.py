@@ -112,3 +112,14 @@ def test_spread_and_traffic_tie(bench, monkeypatch):
 def test_host_info(bench):
     h = bench.host_info()
     assert h["nproc"] == os.cpu_count()
+
+
+def test_measurement_programs_built_and_guarded():
+    """The measurement programs bench.py runs (--stream-ceiling,
+    --filter-bound) are built with the library; filter_bound rejects a
+    lattice its sweep cannot tile before it touches a device."""
+    for name in ("stream_ceiling", "filter_bound"):
+        assert os.access(os.path.join(ROOT, "laph_eigsys_amd", name), os.X_OK), name
+    r = subprocess.run([os.path.join(ROOT, "laph_eigsys_amd", "filter_bound"), "12"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 2 and "multiple of 8" in r.stderr
