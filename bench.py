@@ -83,7 +83,7 @@ def main():
                          "SQ_INSTS_VALU_MFMA_MOPS_F64")
     ap.add_argument("--workload", choices=("c3", "c2", "c4", "fields"), default="c3",
                     help="c3: the headline (32^3x64, N_ev 220); c2: 24^3x48 N_ev 120, every timeslice through the "
-                         "plain drop-in and the batched solver; c4: 48^3x96 N_ev 200 in HBM-sized groups; fields: "
+                         "plain drop-in, the batched solver and the CPU oracle; c4: 48^3x96 N_ev 200 in HBM-sized groups; fields: "
                          "the headline solve on other seeds and field strengths")
     ap.add_argument("--stream-ceiling", action="store_true",
                     help="only run the streaming-bandwidth measurement (laph_eigsys_amd/stream_ceiling)")
@@ -470,7 +470,8 @@ def workload_fields(lp, args):
 def workload_c2(lp, args):
     """24^3 x 48, eps 0.4, HYP 0.62/0.62/3, N_ev 120, DEG 12, tol 1e-10: every
     timeslice through the plain FP64 drop-in (the MatShell path) and through
-    the batched solver; eigenvalues compared timeslice by timeslice."""
+    the batched solver, and (unless --no-cpu) through the oracle on the host
+    cores; eigenvalues compared timeslice by timeslice."""
     L2, T2, NEV2, DEG2 = 24, 48, 120, 12
     cfg = lp.synth_config(1227, 0.4, L2, T2)
     vol = 72 * L2 ** 3
@@ -492,7 +493,7 @@ def workload_c2(lp, args):
     t_shape = time.perf_counter() - t0
     b.close()
     lam_c = 1.35 * max(-e[-1] for e in evb)
-    t_plain, dev, restarts = 0.0, 0.0, []
+    t_plain, dev, restarts, evp = 0.0, 0.0, [], []
     for t in range(T2):
         with lp.Tslice(L2, DEG2, LAMBDA_L, lam_c, ts_config=cfg[t * vol:(t + 1) * vol]) as sl:
             sl.smearing_hyp(0.62, 0.62, 3)
@@ -501,13 +502,45 @@ def workload_c2(lp, args):
             theta, _, pst = sl.eps_solve(NEV2, tol=TOL, want_vectors=False)
             t_plain += time.perf_counter() - tp
         restarts.append(pst.restarts)
-        ev = lp.recover_spectrum(theta, DEG2, LAMBDA_L, lam_c)
-        dev = max(dev, float(np.max(np.abs(ev - evb[t]) / np.abs(evb[t]))))
-    print(json.dumps({"workload": "24^3x48 eps=0.4 N_ev=120 DEG 12 tol 1e-10", "timeslices": T2,
-                      "batched_ts_per_s": T2 / t_batch, "batched_cycles": st.iterations,
-                      "plain_fp64_ts_per_s": T2 / t_plain, "plain_lambda_c": lam_c,
-                      "plain_restarts_max": max(restarts), "max_rel_eigenvalue_difference": dev,
-                      "source_shape_seconds_per_timeslice": t_shape}))
+        evp.append(lp.recover_spectrum(theta, DEG2, LAMBDA_L, lam_c))
+        dev = max(dev, float(np.max(np.abs(evp[t] - evb[t]) / np.abs(evb[t]))))
+    out = {"workload": "24^3x48 eps=0.4 N_ev=120 DEG 12 tol 1e-10", "timeslices": T2,
+           "batched_ts_per_s": T2 / t_batch, "batched_cycles": st.iterations,
+           "plain_fp64_ts_per_s": T2 / t_plain, "plain_lambda_c": lam_c,
+           "plain_restarts_max": max(restarts), "max_rel_eigenvalue_difference": dev,
+           "source_shape_seconds_per_timeslice": t_shape}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_c2(cfg, L2, DEG2, NEV2, lam_c, evp, evb)
+    print(json.dumps(out))
+
+
+def cpu_c2(cfg, L2, deg, nev, lam_c, evp, evb):
+    """The C2 comparison with the CPU run: the oracle (map, HYP, Krylov-Schur
+    on T_deg(B) with the plain drop-in's lambda_c, recover_spectrum; C with
+    OpenMP over the host cores) on every timeslice of the field, its
+    eigenvalues against the plain FP64 drop-in's and the batched solver's,
+    timeslice by timeslice; seconds per timeslice (HYP and solve)."""
+    from oracle import oracle as O
+    vol = 72 * L2 ** 3
+    secs, d_plain, d_batch = [], 0.0, 0.0
+    for t in range(len(evp)):
+        t0 = time.perf_counter()
+        U = O.hyp(O.map_timeslice(cfg[t * vol:(t + 1) * vol], L2), L2, 0.62, 0.62, 3)
+        nconv, theta, _, _ = O.eps_solve(U, L2, LAMBDA_L, lam_c, deg, nev, tol=TOL)
+        secs.append(time.perf_counter() - t0)
+        if nconv < nev:
+            raise RuntimeError(f"c2 cpu: oracle converged {nconv} of {nev} pairs on timeslice {t}")
+        ev = O.recover_spectrum(theta[:nev], deg, LAMBDA_L, lam_c)
+        d_plain = max(d_plain, float(np.max(np.abs(ev - evp[t]) / np.abs(ev))))
+        d_batch = max(d_batch, float(np.max(np.abs(ev - evb[t]) / np.abs(ev))))
+        if t % 8 == 7:
+            print(f"[bench] c2 cpu: {t + 1} timeslices, {sum(secs):.0f}s", file=sys.stderr, flush=True)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": len(secs) / sum(secs), "unit": "timeslices/s", "cores": cores, "kind": "port",
+            "sample": f"all {len(secs)} timeslices, one after another, HYP and Krylov-Schur to {TOL} with OpenMP "
+                      f"over {cores} threads",
+            "seconds_per_timeslice": spread(secs), "max_rel_eigenvalue_difference_vs_plain_fp64": d_plain,
+            "max_rel_eigenvalue_difference_vs_batched": d_batch, **host_info()}
 
 
 def workload_c4(lp, args):
