@@ -125,6 +125,7 @@ static int batch_alloc(laph_batch* b) {
     laph_lane& ln = b->lanes[h];
     ln.t0 = (int)((long long)h * b->nts / nlanes);
     ln.nt = (int)((long long)(h + 1) * b->nts / nlanes) - ln.t0;
+    ln.wcap = 0.75 * b->p.lambda_l;
     LAPH_HIP(hipStreamCreateWithFlags(&ln.st, hipStreamNonBlocking));
     LAPH_CHECK(rocblas_create_handle(&ln.rb) == rocblas_status_success &&
                    rocblas_set_stream(ln.rb, ln.st) == rocblas_status_success,
@@ -539,8 +540,9 @@ static int rayleigh_ritz(const View& v, Act q, bool orthonormal, bool* chol_fail
                                         zc(T + q.l), K, (rocblas_stride)b->per_ts, nt));
   // residuals of the new Ritz pairs for the next filter pass (D X is not kept:
   // the next Rayleigh-Ritz applies D to the filtered block)
-  LAPH_HIP(launch_lap_residual64(b->U64 + v.lo(), T, b->mu + v.nk(), b->p.lambda_l, fb(b, b->R32, v.vo()), b->p.L,
-                                 K, nt, b->lock_blk + v.ln->t0, b->strip, b->bw, b->bf16, stencil_zrun(b), v.ln->st));
+  LAPH_HIP(launch_lap_residual64(b->U64 + v.lo(), T, b->mu + v.nk(), b->p.lambda_l, v.ln->wcap,
+                                 fb(b, b->R32, v.vo()), b->p.L, K, nt, b->lock_blk + v.ln->t0, b->strip, b->bw,
+                                 b->bf16, stencil_zrun(b), v.ln->st));
   // the hard-locked columns of X live in the old X buffer: carry them over
   if (q.l > 0)
     LAPH_HIP(hipMemcpy2DAsync(T, (size_t)K * sizeof(double2), Y, (size_t)K * sizeof(double2),
@@ -613,9 +615,9 @@ static int start_from_block(const View& v) {
   std::fill(b->h_mu.begin() + v.nk(), b->h_mu.begin() + v.nk() + nk, b->window0);
   LAPH_HIP(hipMemcpyAsync(b->mu + v.nk(), b->h_mu.data() + v.nk(), nk * sizeof(double), hipMemcpyHostToDevice,
                           v.ln->st));
-  LAPH_HIP(launch_lap_residual64(b->U64 + v.lo(), v.X(), b->mu + v.nk(), b->p.lambda_l, fb(b, b->R32, v.vo()),
-                                 b->p.L, b->K, v.ln->nt, b->lock_blk + v.ln->t0, b->strip, b->bw, b->bf16,
-                                 stencil_zrun(b), v.ln->st));
+  LAPH_HIP(launch_lap_residual64(b->U64 + v.lo(), v.X(), b->mu + v.nk(), b->p.lambda_l, v.ln->wcap,
+                                 fb(b, b->R32, v.vo()), b->p.L, b->K, v.ln->nt, b->lock_blk + v.ln->t0, b->strip,
+                                 b->bw, b->bf16, stencil_zrun(b), v.ln->st));
   v.ln->sv64 += (long long)v.ln->nt * b->K;
   return LAPH_OK;
 }
@@ -631,10 +633,10 @@ static int residuals_and_window(const View& v, std::vector<double>& lc, int* n_a
     // lower end of the damped interval: largest Ritz value of -D, kept
     // well below lambda_l so the amplification stays inside FP32 range (the
     // residual kernels derive the same window from the device copy of mu)
-    lc[t] = std::min(b->h_mu[v.nk() + (size_t)t * K + K - 1], 0.75 * lamL);
+    lc[t] = std::min(b->h_mu[v.nk() + (size_t)t * K + K - 1], ln->wcap);
   }
-  LAPH_HIP(launch_residual_norms(fb(b, b->R32, v.vo()), b->mu + v.nk(), lamL, b->nrm2 + v.nk(), v.red(), b->n, K, nt,
-                                 ln->lock, b->bw, b->bf16, ln->st));
+  LAPH_HIP(launch_residual_norms(fb(b, b->R32, v.vo()), b->mu + v.nk(), lamL, ln->wcap, b->nrm2 + v.nk(), v.red(),
+                                 b->n, K, nt, ln->lock, b->bw, b->bf16, ln->st));
   LAPH_HIP(hipMemcpyAsync(b->h_res.data() + v.nk(), b->nrm2 + v.nk(), (size_t)nt * K * sizeof(double),
                           hipMemcpyDeviceToHost, ln->st));
   LAPH_HIP(hipStreamSynchronize(ln->st));
@@ -1139,7 +1141,17 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
   }
   // start block (network proposal or seeded Gaussian), CholQR2, first Rayleigh-Ritz
   if (!warm) LAPH_HIP(launch_random_block(v.X(), (size_t)nt * b->per_ts, b->o.seed + 0x1000193ull * ln->t0, ln->st));
-  if (warm || b->start_rr || b->qr_cycles > 0 || b->warm_cycles == 0)
+  const bool start_rr = warm || b->start_rr || b->qr_cycles > 0 || b->warm_cycles == 0;
+  const bool warmup0 = !(warm && b->o.warm_start == 2) && b->warm_cycles > 0;
+  // The warm-up cycle after a Rayleigh-Ritz start keeps its window's lower end
+  // at most at the start window: the largest Ritz value of a start block that
+  // is not yet filtered (a random block, or a proposal only partly inside the
+  // wanted space) sits high in the spectrum, and a window that high makes the
+  // warm-up filter a weak low-pass.  The cap holds for the residuals written
+  // by the start's Rayleigh-Ritz step and for the filter of that cycle.
+  ln->wcap = start_rr && warmup0 && b->warmup_cap ? std::min(b->window0, 0.75 * b->p.lambda_l)
+                                                 : 0.75 * b->p.lambda_l;
+  if (start_rr)
     LAPH_TRY(orthonormalise_and_rr(v, Act{0, K}, b->qr_cycles > 0));
   else
     LAPH_TRY(start_from_block(v));
@@ -1190,6 +1202,7 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
     LAPH_HIP(hipEventRecord(ln->ev[0], ln->st));
     LAPH_TRY(run_filter(v, passes, &w));
     LAPH_HIP(hipEventRecord(ln->ev[1], ln->st));
+    ln->wcap = 0.75 * b->p.lambda_l;  // later windows: the Ritz values (the next residuals use this cap)
     dim3 g((unsigned)((b->per_ts + 255) / 256), nt);
     if (b->bf16)
       k_combine_gamma<unsigned><<<g, 256, 0, ln->st>>>(v.X(), static_cast<const unsigned*>(w), b->gam + v.nk(),
@@ -1586,6 +1599,7 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
       const double w = std::atof(e);
       if (w > 0.0) b->window0 = std::min(w, 0.75 * p->lambda_l);
     }
+    if (const char* e = std::getenv("LAPH_WARMUP_WINDOW_CAP")) b->warmup_cap = std::atoi(e) != 0;
     b->roots = laph::leja_roots(p->deg);
     b->strip = strip > 0 ? strip : p->L;
     b->strip_fixed = strip > 0;

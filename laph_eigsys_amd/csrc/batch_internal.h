@@ -33,6 +33,8 @@ struct laph_lane {
   hipGraphExec_t graph = nullptr;  // correction-form filter pass of the batch's degree
   std::map<int, hipGraphExec_t> gdeg;  // correction passes by degree (per-cycle degree mode)
   int m = 0;                    // filter degree of the current cycle (0: the batch's degree)
+  double wcap = 0.0;            // cap of the window's lower end: 0.75 lambda_l, or the start window
+                                // in a warm-up cycle that follows a Rayleigh-Ritz start
   hipGraphExec_t gplain[LAPH_MAX_PASSES - 1] = {};  // plain passes 2 .. LAPH_MAX_PASSES
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   // per-solve statistics
@@ -100,6 +102,8 @@ struct laph_batch {
   int bf16 = 0;  // filter vectors stored as BF16 (laph_batch_opts.filter_storage)
   int warm_cycles = 1;  // cycles of the batch's degree while the window settles (LAPH_WARM)
   double window0 = 0;   // window lower end of the first cycle from the start block (LAPH_WINDOW0)
+  bool warmup_cap = true;  // warm-up cycle after a Rayleigh-Ritz start: window at most window0
+                          // (LAPH_WARMUP_WINDOW_CAP=0: from the start block's Ritz values)
   int warm_deg = 0;     // degree of the warm-up cycles in the per-cycle degree mode (0: the batch's degree)
   int strip = 0;  // sweep strip width of the stencil kernels (divides L; L = x-major planes)
   bool strip_fixed = false;  // LAPH_STRIP given

@@ -32,11 +32,12 @@ hipError_t launch_lap_block64(const double2* U64, const double2* v, double2* out
                               const int* first_blk, int W, int zrun, hipStream_t st);
 // Residuals of the Ritz pairs from the FP64 stencil: r = c_t (D X + X diag(mu))
 // in the filter storage (block-major, bw-vector blocks, FP32 or BF16), mu the
-// Ritz values of -D ([nts][K]), c_t = 2 / (lamL - min(mu[t][K-1], 0.75 lamL)),
-// for the 64-column blocks >= first_blk[ts].
-hipError_t launch_lap_residual64(const double2* U64, const double2* X, const double* mu, double lamL, void* r,
-                                 int L, int K, int nts, const int* first_blk, int W, int bw, int bf16, int zrun,
-                                 hipStream_t st);
+// Ritz values of -D ([nts][K]), c_t = 2 / (lamL - min(mu[t][K-1], wcap)) (wcap
+// the cap of the window's lower end, 0.75 lamL outside a warm-up cycle that
+// starts from a Rayleigh-Ritz step), for the 64-column blocks >= first_blk[ts].
+hipError_t launch_lap_residual64(const double2* U64, const double2* X, const double* mu, double lamL, double wcap,
+                                 void* r, int L, int K, int nts, const int* first_blk, int W, int bw, int bf16,
+                                 int zrun, hipStream_t st);
 // Column sums over a block's rows go through per-chunk partials of kRedRows
 // rows, added in chunk order (deterministic: no atomics); `red` holds
 // nts * red_chunks(n) * K doubles.
@@ -44,8 +45,9 @@ constexpr int kRedRows = 512;
 inline int red_chunks(int n) { return (n + kRedRows - 1) / kRedRows; }
 // nrm2 = column sums of |D X + X diag(mu)|^2 from that r, for the columns
 // >= first_col (a multiple of 64).
-hipError_t launch_residual_norms(const void* r, const double* mu, double lamL, double* nrm2, double* red, int n,
-                                 int K, int nts, int first_col, int bw, int bf16, hipStream_t st);
+hipError_t launch_residual_norms(const void* r, const double* mu, double lamL, double wcap, double* nrm2,
+                                 double* red, int n, int K, int nts, int first_col, int bw, int bf16,
+                                 hipStream_t st);
 // fix_phase (eigensystem.cpp:4-30) in place on the first nev columns of every
 // timeslice of a [ts][n][K] block: phase[ts][col] = arg X[ts][0][col], then
 // each column times e^{-i phase}.

@@ -811,9 +811,10 @@ __global__ __launch_bounds__(256) void k_lap_block64(const double2* __restrict__
 }
 
 // c_t = 2 / (lambda_l - lc_t) of a timeslice, lc_t = min(largest Ritz value
-// of -D, 0.75 lambda_l): the scale of B = c_t D + a_t in the current window.
-__device__ __forceinline__ double window_scale(const double* __restrict__ mu_ts, int K, double lamL) {
-  return 2.0 / (lamL - fmin(mu_ts[K - 1], 0.75 * lamL));
+// of -D, wcap): the scale of B = c_t D + a_t in the current window (wcap is
+// 0.75 lambda_l, lower for the warm-up cycle of a Rayleigh-Ritz start).
+__device__ __forceinline__ double window_scale(const double* __restrict__ mu_ts, int K, double lamL, double wcap) {
+  return 2.0 / (lamL - fmin(mu_ts[K - 1], wcap));
 }
 
 // Residuals of the Ritz pairs straight from the FP64 stencil: R = D X + X
@@ -822,7 +823,7 @@ __device__ __forceinline__ double window_scale(const double* __restrict__ mu_ts,
 // 64-column blocks >= first_blk[ts].  D X itself is not kept.
 __global__ __launch_bounds__(256) void k_lap_residual64(const double2* __restrict__ U64,
                                                          const double2* __restrict__ v,
-                                                         const double* __restrict__ mu, double lamL,
+                                                         const double* __restrict__ mu, double lamL, double wcap,
                                                          void* __restrict__ rbuf, int L, int K, int nts,
                                                          const int* __restrict__ first_blk, int W, int bw,
                                                          int bf16) {
@@ -839,7 +840,7 @@ __global__ __launch_bounds__(256) void k_lap_residual64(const double2* __restric
   double2 acc[3], ctr[3];
   hop<double2>(U64 + (size_t)ts * V3 * 27, v + vo, s, L, K, col, acc, ctr);
   const double* m = mu + (size_t)ts * K;
-  const double c = window_scale(m, K, lamL), mc = m[col];
+  const double c = window_scale(m, K, lamL, wcap), mc = m[col];
   const size_t n = (size_t)3 * V3;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -867,7 +868,8 @@ __global__ __launch_bounds__(256) void k_lap_residual64(const double2* __restric
 template <bool RES, int BW, bool BF16, int NZ, bool LINE>
 __global__ __launch_bounds__(256) void k_lap64y(const double2* __restrict__ U64, const double2* __restrict__ v,
                                                 double2* __restrict__ out, const double* __restrict__ mu,
-                                                double lamL, void* __restrict__ rbuf, int L, int K, int nts,
+                                                double lamL, double wcap, void* __restrict__ rbuf, int L, int K,
+                                                int nts,
                                                 const int* __restrict__ first_blk, int W) {
   __shared__ double2 rows[2][4][3][64];
   const int V3 = L * L * L;
@@ -896,7 +898,7 @@ __global__ __launch_bounds__(256) void k_lap64y(const double2* __restrict__ U64,
   double cs = 0.0, mc = 0.0;
   if (RES) {
     const double* m = mu + (size_t)ts * K;
-    cs = window_scale(m, K, lamL);
+    cs = window_scale(m, K, lamL, wcap);
     mc = m[cb * 64 + l64] - 6.0;
   }
   double2 zm[3], zc[3];
@@ -1001,14 +1003,14 @@ __global__ __launch_bounds__(256) void k_lap64y(const double2* __restrict__ U64,
 // chunk partials are summed in chunk order by k_sum_chunks: no atomics, so a
 // repeated solve reproduces every bit.
 __global__ __launch_bounds__(256) void k_residual_norms(const void* __restrict__ rbuf,
-                                                         const double* __restrict__ mu, double lamL,
+                                                         const double* __restrict__ mu, double lamL, double wcap,
                                                          double* __restrict__ red, int n, int K,
                                                          int first_col, int bw, int bf16) {
   const int ts = blockIdx.z;
   const int col = first_col + blockIdx.y * 64 + (threadIdx.x & 63);
   const int r0 = blockIdx.x * kRedRows + (threadIdx.x >> 6);
   const int r1 = min(n, (int)(blockIdx.x + 1) * kRedRows);
-  const double c = window_scale(mu + (size_t)ts * K, K, lamL);
+  const double c = window_scale(mu + (size_t)ts * K, K, lamL, wcap);
   const size_t base = ((size_t)ts * (K / bw) + col / bw) * n * bw + col % bw;
   double acc = 0.0;
   for (int row = r0; row < r1; row += 4) {
@@ -1256,21 +1258,22 @@ hipError_t launch_filter_step(const float2* U32, const void* in_, const void* r_
 // whole lines), the residual kernel whole z-lines.
 template <bool RES, int BW, bool BF16>
 static hipError_t launch_lap64y(const double2* U64, const double2* v, double2* out, const double* mu, double lamL,
-                                void* r, int L, int K, int nts, const int* first_blk, int W, int zrun, hipStream_t st) {
+                                double wcap, void* r, int L, int K, int nts, const int* first_blk, int W, int zrun,
+                                hipStream_t st) {
   const int zspan = RES ? L : zrun;
   const dim3 g((unsigned)(L * L * L / (4 * zspan)) * (unsigned)(K / 64) * (unsigned)nts);
   switch (zrun) {
     case 1:
-      k_lap64y<RES, BW, BF16, 1, RES><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W);
+      k_lap64y<RES, BW, BF16, 1, RES><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, wcap, r, L, K, nts, first_blk, W);
       break;
     case 2:
-      k_lap64y<RES, BW, BF16, 2, RES><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W);
+      k_lap64y<RES, BW, BF16, 2, RES><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, wcap, r, L, K, nts, first_blk, W);
       break;
     case 4:
-      k_lap64y<RES, BW, BF16, 4, RES><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W);
+      k_lap64y<RES, BW, BF16, 4, RES><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, wcap, r, L, K, nts, first_blk, W);
       break;
     case 8:
-      k_lap64y<RES, BW, BF16, 8, RES><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, r, L, K, nts, first_blk, W);
+      k_lap64y<RES, BW, BF16, 8, RES><<<g, 256, 0, st>>>(U64, v, out, mu, lamL, wcap, r, L, K, nts, first_blk, W);
       break;
     default:
       return hipErrorInvalidValue;
@@ -1282,39 +1285,42 @@ hipError_t launch_lap_block64(const double2* U64, const double2* v, double2* out
                               const int* first_blk, int W, int zrun, hipStream_t st) {
   if (zrun > 0) {
     if (L % zrun != 0 || W % 4 != 0) return hipErrorInvalidValue;
-    return launch_lap64y<false, 64, false>(U64, v, out, nullptr, 0.0, nullptr, L, K, nts, first_blk, W, zrun, st);
+    return launch_lap64y<false, 64, false>(U64, v, out, nullptr, 0.0, 0.0, nullptr, L, K, nts, first_blk, W, zrun,
+                                                   st);
   }
   k_lap_block64<<<site_grid(L, K, nts), 256, 0, st>>>(U64, v, out, L, K, nts, first_blk, W);
   return hipGetLastError();
 }
 
-hipError_t launch_lap_residual64(const double2* U64, const double2* X, const double* mu, double lamL, void* r,
-                                 int L, int K, int nts, const int* first_blk, int W, int bw, int bf16, int zrun,
-                                 hipStream_t st) {
+hipError_t launch_lap_residual64(const double2* U64, const double2* X, const double* mu, double lamL, double wcap,
+                                 void* r, int L, int K, int nts, const int* first_blk, int W, int bw, int bf16,
+                                 int zrun, hipStream_t st) {
   if (zrun > 0) {
     if (L % zrun != 0 || W % 4 != 0) return hipErrorInvalidValue;
     if (bf16 && bw == 256)
-      return launch_lap64y<true, 256, true>(U64, X, nullptr, mu, lamL, r, L, K, nts, first_blk, W, zrun, st);
+      return launch_lap64y<true, 256, true>(U64, X, nullptr, mu, lamL, wcap, r, L, K, nts, first_blk, W, zrun, st);
     if (bf16 && bw == 128)
-      return launch_lap64y<true, 128, true>(U64, X, nullptr, mu, lamL, r, L, K, nts, first_blk, W, zrun, st);
+      return launch_lap64y<true, 128, true>(U64, X, nullptr, mu, lamL, wcap, r, L, K, nts, first_blk, W, zrun, st);
     if (!bf16 && bw == 128)
-      return launch_lap64y<true, 128, false>(U64, X, nullptr, mu, lamL, r, L, K, nts, first_blk, W, zrun, st);
+      return launch_lap64y<true, 128, false>(U64, X, nullptr, mu, lamL, wcap, r, L, K, nts, first_blk, W, zrun, st);
     if (!bf16 && bw == 64)
-      return launch_lap64y<true, 64, false>(U64, X, nullptr, mu, lamL, r, L, K, nts, first_blk, W, zrun, st);
+      return launch_lap64y<true, 64, false>(U64, X, nullptr, mu, lamL, wcap, r, L, K, nts, first_blk, W, zrun, st);
     if (!bf16 && bw == 32)
-      return launch_lap64y<true, 32, false>(U64, X, nullptr, mu, lamL, r, L, K, nts, first_blk, W, zrun, st);
+      return launch_lap64y<true, 32, false>(U64, X, nullptr, mu, lamL, wcap, r, L, K, nts, first_blk, W, zrun, st);
     return hipErrorInvalidValue;
   }
-  k_lap_residual64<<<site_grid(L, K, nts), 256, 0, st>>>(U64, X, mu, lamL, r, L, K, nts, first_blk, W, bw, bf16);
+  k_lap_residual64<<<site_grid(L, K, nts), 256, 0, st>>>(U64, X, mu, lamL, wcap, r, L, K, nts, first_blk, W, bw,
+                                                         bf16);
   return hipGetLastError();
 }
 
-hipError_t launch_residual_norms(const void* r, const double* mu, double lamL, double* nrm2, double* red, int n,
-                                 int K, int nts, int first_col, int bw, int bf16, hipStream_t st) {
+hipError_t launch_residual_norms(const void* r, const double* mu, double lamL, double wcap, double* nrm2,
+                                 double* red, int n, int K, int nts, int first_col, int bw, int bf16,
+                                 hipStream_t st) {
   const int w = K - first_col;
   const int nchunk = red_chunks(n);
   dim3 g(nchunk, w / 64, nts);
-  k_residual_norms<<<g, 256, 0, st>>>(r, mu, lamL, red, n, K, first_col, bw, bf16);
+  k_residual_norms<<<g, 256, 0, st>>>(r, mu, lamL, wcap, red, n, K, first_col, bw, bf16);
   k_sum_chunks<<<(nts * w + 255) / 256, 256, 0, st>>>(red, nrm2, nts, K, first_col, nchunk);
   return hipGetLastError();
 }

@@ -450,3 +450,40 @@ def test_layout_cache(lp, gpu, tmp_path, monkeypatch):
     st3, _ = run(nsub=192)  # another search space: a miss, timed and appended
     assert st3.layout_from_cache == 0
     assert len(cache.read_text().splitlines()) == 3
+
+
+def test_rayleigh_ritz_start_keeps_the_start_window(lp, gpu, monkeypatch):
+    """A warm-up cycle that follows a Rayleigh-Ritz step on the start block
+    (LAPH_START_RR=1; a network proposal takes the same path) keeps its
+    window's lower end at the start window, as the start straight from the
+    block does: both take the same cycles (and within 5% the same stencil
+    applications) to the plain FP64 drop-in's spectrum.  With the window taken
+    from the unfiltered block's own Ritz values instead
+    (LAPH_WARMUP_WINDOW_CAP=0) the warm-up filter damps only the top of the
+    spectrum and the solve needs more work."""
+    L, nts, nev, deg, tol = 16, 2, 60, 12, 1e-10
+    data = smeared(L, 31, 0.4, nts)
+    runs = {}
+    for name, env in (("block", {"LAPH_START_RR": "0"}), ("rr", {"LAPH_START_RR": "1"}),
+                      ("rr_uncapped", {"LAPH_START_RR": "1", "LAPH_WARMUP_WINDOW_CAP": "0"})):
+        for k, val in env.items():
+            monkeypatch.setenv(k, val)
+        b = lp.Batch(L, deg, 12.0, 0.0, nts, nev, nsub=128, tol=tol, seed=4, device=gpu)
+        for t, (ts, _) in enumerate(data):
+            b.set_timeslice(t, ts, 0.62, 0.62, 3)
+        st = b.solve()
+        assert st.converged == nts
+        runs[name] = (st.iterations, st.stencil_vecs, [b.get(t, want_vectors=False)[0] for t in range(nts)])
+        b.close()
+        monkeypatch.delenv("LAPH_WARMUP_WINDOW_CAP", raising=False)
+    print({k: v[:2] for k, v in runs.items()})  # (cycles, FP32 stencil applications)
+    assert runs["rr"][0] == runs["block"][0] and abs(runs["rr"][1] - runs["block"][1]) <= 0.05 * runs["block"][1]
+    assert runs["rr_uncapped"][:2] >= runs["rr"][:2]  # measured at 16^3: 4 cycles, 31744 against 30720
+    for t, (ts, _) in enumerate(data):  # reference: the plain FP64 drop-in
+        lamC = -runs["block"][2][t][-1] * 1.3
+        with lp.Tslice(L, deg, 12.0, lamC, device=gpu, ts_config=ts) as tsl:
+            tsl.smearing_hyp(0.62, 0.62, 3)
+            theta, _, _ = tsl.eps_solve(nev, tol=1e-12, want_vectors=False)
+        evp = lp.recover_spectrum(theta, deg, 12.0, lamC)
+        for name in runs:
+            np.testing.assert_allclose(runs[name][2][t], evp, rtol=1e-10, atol=1e-12, err_msg=name)
