@@ -5,8 +5,8 @@ A step is one batched eigensolve of all 64 timeslices of the seeded
 synthetic configuration (U = exp(i 0.3 H), HYP 0.62/0.62/3 applied on the
 device beforehand; smeared links resident in HBM when the timed region
 starts): FP32 Chebyshev filter in residual-correction form (one warm-up
-cycle of degree 16, then one filter per cycle of the degree the window
-allows, up to 72) on 64- or 128-vector blocks, FP64 Rayleigh-Ritz / residual
+cycle of degree 16 or the higher degree the first, untimed solve found
+faster, then one filter per cycle of the degree the window allows, up to 72) on 64- or 128-vector blocks, FP64 Rayleigh-Ritz / residual
 locking, until every timeslice has 220 locked pairs with
 |D v - lambda v| <= 1e-10, then fix_phase of every eigenvector: the timed
 region ends with phase-fixed eigenpairs resident in HBM.  A timed step that
@@ -230,10 +230,12 @@ def main():
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32 filter + fp64 CholQR2/RR/lock", "data": "synthetic",
         "config": {"workload": f"{L}^3x{nts} eps={EPS} HYP 0.62/0.62/3, N_ev={NEV}, Chebyshev degree {DEG}, "
-                               f"one filter pass per cycle of degree <= 72 after one of degree {DEG}, vector block "
+                               f"one filter pass per cycle of degree <= 72 after one of degree {st.warmup_degree}, "
+                               f"vector block "
                                f"{st.filter_block}, sweep strip {st.filter_strip}, search space {NSUB}, "
                                f"residual <= {TOL}, ending with phase-fixed eigenpairs in HBM",
-                   "lattice": f"{L}^3x{nts}", "nev": NEV, "deg": DEG, "tol": TOL, "parallelism": "none (1 GPU)"},
+                   "lattice": f"{L}^3x{nts}", "nev": NEV, "deg": DEG, "warmup_degree": st.warmup_degree, "tol": TOL,
+                   "parallelism": "none (1 GPU)"},
         "roofline": roofline,
         "gram": gram,
         "plain_fp64": {"value": plain_value, "unit": "timeslices/s", "timeslices": len(plain_times),
@@ -506,6 +508,7 @@ def workload_c2(lp, args):
         dev = max(dev, float(np.max(np.abs(evp[t] - evb[t]) / np.abs(evb[t]))))
     out = {"workload": "24^3x48 eps=0.4 N_ev=120 DEG 12 tol 1e-10", "timeslices": T2,
            "batched_ts_per_s": T2 / t_batch, "batched_cycles": st.iterations,
+           "batched_warmup_degree": st.warmup_degree,
            "plain_fp64_ts_per_s": T2 / t_plain, "plain_lambda_c": lam_c,
            "plain_restarts_max": max(restarts), "max_rel_eigenvalue_difference": dev,
            "source_shape_seconds_per_timeslice": t_shape}
@@ -626,6 +629,7 @@ def workload_c4(lp, args):
                       "write_GBps": wbytes / t_write / 1e9, "write_seconds_per_timeslice": t_write / 2,
                       "groups": [n for _, n in groups], "ts_per_s_solve": T4 / t_solve,
                       "ts_per_s_with_upload_hyp": T4 / (t_solve + t_set), "cycles_max": cycles,
+                      "warmup_degree": st.warmup_degree,
                       "max_residual": maxres, "filter_block": st.filter_block, "filter_strip": st.filter_strip,
                       "filter_GBps": by / ms / 1e6, "filter_us_per_launch": ms * 1e3, "gram": gram}))
 

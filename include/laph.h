@@ -287,6 +287,10 @@ typedef struct laph_batch_stats {
                               64 x 64 tiles of the upper triangle), 0: on rocBLAS zgemm */
   int layout_from_cache;   /* 1: the filter and Gram layouts were read from the LAPH_TUNE_FILE
                               cache instead of timed on the first solve */
+  int warmup_degree;       /* Chebyshev degree of the warm-up cycle: on the first solve from the
+                              start block DEG and up to three higher degrees (24, 32, 40, within
+                              the amplification bound at the start window) are each solved and
+                              timed, and the fastest is kept (LAPH_WARM_DEG fixes it) */
 } laph_batch_stats;
 
 int laph_batch_create(laph_batch** out, const laph_params* params, const laph_batch_opts* opts);

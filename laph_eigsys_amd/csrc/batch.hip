@@ -1005,9 +1005,11 @@ static int run_filter(const View& v, int passes, void** w) {
   // the batch's own degree uses ln->graph; other degrees (per-cycle degree
   // mode) a graph each, captured on first use
   hipGraphExec_t* g = v.m() == v.b->deg ? &ln->graph : &ln->gdeg[v.m()];
+  const auto c0 = std::chrono::steady_clock::now();  // host time of first-use captures (warm-up degree timing)
   if (!*g) LAPH_TRY(capture(v, g, [&] { return enqueue_filter(v); }));
   for (int p = 2; p <= passes; ++p)
     if (!ln->gplain[p - 2]) LAPH_TRY(capture(v, &ln->gplain[p - 2], [&] { return enqueue_plain_pass(v, p); }));
+  ln->s_capture += std::chrono::duration<double>(std::chrono::steady_clock::now() - c0).count();
   // the steps' links: c_t U for this cycle's window of every timeslice
   LAPH_HIP(launch_scale_links32(v.b->U32 + v.lo(), v.b->cab + ln->t0, v.b->U32c + v.lo(), (size_t)v.b->V3 * 27,
                                 ln->nt, ln->st));
@@ -1135,6 +1137,7 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
   ln->rr_fallbacks = ln->shifted = 0;
   ln->fbytes = 0;
   ln->ms_filter = ln->ms_dense = 0;
+  ln->s_capture = 0;
   {
     std::vector<int> zero(nt, 0);
     LAPH_HIP(hipMemcpyAsync(b->lock_blk + ln->t0, zero.data(), nt * sizeof(int), hipMemcpyHostToDevice, ln->st));
@@ -1291,18 +1294,42 @@ static int time_correction_pass(laph_batch* b, int reps, float* ms, float reject
 // Layout cache (LAPH_TUNE_FILE): the layouts the first solve chose, one text
 // line per shape and device,
 //   "<arch> cu<CUs> L<L> K<K> nt<lane 0 timeslices> s<bf16> f<prod><qform> z<zrun>
-//    : bw <width> strip <strip> split <row blocks> chunks <site chunks> mfma <0|1>"
+//    d<DEG> n<nev> t<tol> : bw <width> strip <strip> split <row blocks>
+//    chunks <site chunks> mfma <0|1> warm <warm-up degree>"
 // read before timing (the first valid line of the key wins) and appended
 // after a miss, so that later processes on the same shape skip the timing
 // (several seconds at 32^3 x 64) and pick the same layout.  Used only while no
-// layout control (LAPH_BW, LAPH_STRIP, LAPH_GRAM_SPLIT, LAPH_GRAM_MFMA) is set.
+// layout control (LAPH_BW, LAPH_STRIP, LAPH_GRAM_SPLIT, LAPH_GRAM_MFMA,
+// LAPH_WARM_DEG) is set.
 static std::string tune_key(const laph_batch* b) {
   hipDeviceProp_t pr;
   if (hipGetDeviceProperties(&pr, b->p.device) != hipSuccess) return std::string();
-  char k[320];
-  std::snprintf(k, sizeof k, "%s cu%d L%d K%d nt%d s%d f%d%d z%d", pr.gcnArchName, pr.multiProcessorCount, b->p.L,
-                b->K, b->lanes[0].nt, (int)b->bf16, (int)b->prod, (int)b->qform, b->zrun);
+  char k[384];
+  std::snprintf(k, sizeof k, "%s cu%d L%d K%d nt%d s%d f%d%d z%d d%d n%d t%g", pr.gcnArchName,
+                pr.multiProcessorCount, b->p.L, b->K, b->lanes[0].nt, (int)b->bf16, (int)b->prod, (int)b->qform,
+                b->zrun, b->deg, b->nev, b->o.tol);
   return std::string(k);
+}
+
+// Warm-up degrees the first solve times (per-cycle degree mode with a warm-up
+// cycle, LAPH_WARM_DEG unset): 24, 32 and 40 above the configuration's degree,
+// each at most the highest degree whose amplification at the start window stays
+// under amp_cap (that degree itself when it is below 40), highest first, then
+// the configuration's degree.  A higher warm-up degree can save a whole cycle
+// (24^3: 5 -> 4, 48^3: 6 -> 5) or only cost filter steps (32^3: 4 cycles at
+// every degree), so it is measured rather than set.  Empty: nothing to choose.
+static std::vector<int> warm_candidates(const laph_batch* b) {
+  std::vector<int> c;
+  if (b->fdeg_max <= 0 || b->warm_cycles <= 0 || b->warm_deg_fixed) return c;
+  const double w = b->window0, lamL = b->p.lambda_l;
+  const double p0 = std::acosh(1.0 + 2.0 * w / (lamL - w));
+  int dcap = p0 > 0.0 ? (int)std::floor(std::acosh(b->amp_cap) / p0) : b->fdeg_max;
+  dcap = std::min(dcap, b->fdeg_max) / 4 * 4;
+  for (int d : {std::min(40, dcap), 32, 24})
+    if (d > b->deg && d <= dcap && std::find(c.begin(), c.end(), d) == c.end()) c.push_back(d);
+  if (c.empty()) return c;
+  c.push_back(b->deg);
+  return c;
 }
 
 static bool tune_lookup(laph_batch* b, const char* path, const std::string& key) {
@@ -1314,18 +1341,25 @@ static bool tune_lookup(laph_batch* b, const char* path, const std::string& key)
   while (!hit && std::fgets(line, sizeof line, f)) {
     const char* sep = std::strstr(line, " : ");
     if (!sep || (size_t)(sep - line) != key.size() || std::strncmp(line, key.c_str(), key.size()) != 0) continue;
-    int bw = 0, strip = 0, split = 0, ks = 0, mf = 0;
-    if (std::sscanf(sep + 3, "bw %d strip %d split %d chunks %d mfma %d", &bw, &strip, &split, &ks, &mf) != 5)
+    int bw = 0, strip = 0, split = 0, ks = 0, mf = 0, wd = 0;
+    if (std::sscanf(sep + 3, "bw %d strip %d split %d chunks %d mfma %d warm %d", &bw, &strip, &split, &ks, &mf,
+                    &wd) != 6)
       continue;
+    const std::vector<int> wc = warm_candidates(b);
     const bool ok = std::find(cand.begin(), cand.end(), bw) != cand.end() && b->K % bw == 0 && strip >= 1 &&
                     b->p.L % strip == 0 && split >= 1 && split <= 8 && ks >= 1 && ks <= b->gram_ksplit_max &&
-                    b->n % ks == 0 && (mf == 0 || (mf == 1 && split == 1 && gram_mfma_fits(b->n, b->K, ks)));
+                    b->n % ks == 0 && (mf == 0 || (mf == 1 && split == 1 && gram_mfma_fits(b->n, b->K, ks))) &&
+                    (wc.empty() || std::find(wc.begin(), wc.end(), wd) != wc.end());
     if (!ok) continue;
     b->bw = bw;
     b->strip = strip;
     b->gram_split = split;
     b->gram_ksplit = ks;
     b->gram_mfma = mf;
+    if (!wc.empty()) {
+      b->warm_deg = wd;
+      b->warm_tuned = true;
+    }
     hit = true;
   }
   std::fclose(f);
@@ -1335,8 +1369,8 @@ static bool tune_lookup(laph_batch* b, const char* path, const std::string& key)
 static void tune_store(const laph_batch* b, const char* path, const std::string& key) {
   FILE* f = std::fopen(path, "a");
   if (!f) return;  // an unwritable cache only costs the next process the timing
-  std::fprintf(f, "%s : bw %d strip %d split %d chunks %d mfma %d\n", key.c_str(), b->bw, b->strip, b->gram_split,
-               b->gram_ksplit, b->gram_mfma);
+  std::fprintf(f, "%s : bw %d strip %d split %d chunks %d mfma %d warm %d\n", key.c_str(), b->bw, b->strip,
+               b->gram_split, b->gram_ksplit, b->gram_mfma, b->warm_deg > 0 ? b->warm_deg : b->deg);
   std::fclose(f);
 }
 
@@ -1351,7 +1385,7 @@ static void tune_store(const laph_batch* b, const char* path, const std::string&
 static int autotune_block_width(laph_batch* b) {
   laph_lane* ln = &b->lanes[0];
   const char* tf = std::getenv("LAPH_TUNE_FILE");
-  const bool cache = tf && *tf && b->bw == 0 && !b->strip_fixed && !b->gram_split_fixed;
+  const bool cache = tf && *tf && b->bw == 0 && !b->strip_fixed && !b->gram_split_fixed && !b->warm_deg_fixed;
   const std::string key = cache ? tune_key(b) : std::string();
   if (!key.empty() && tune_lookup(b, tf, key)) {
     b->tuned = b->tuned_from_cache = true;
@@ -1432,7 +1466,7 @@ static int autotune_block_width(laph_batch* b) {
   }
   LAPH_HIP(hipGraphExecDestroy(ln->graph));  // captured for the last layout timed
   ln->graph = nullptr;
-  if (!key.empty()) tune_store(b, tf, key);
+  b->tune_miss_key = key;  // appended once the first solve has chosen the warm-up degree
   if (b->trace)
     std::fprintf(stderr, "[laph] filter block width %d, strip %d, Gram %s, row blocks %d, site chunks %d\n", wbest,
                  sbest, b->gram_mfma ? "MFMA kernel" : "rocBLAS", b->gram_split, b->gram_ksplit);
@@ -1447,13 +1481,8 @@ int output_buffers(laph_batch* b) {
   return LAPH_OK;
 }
 
-static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
-  auto t0 = std::chrono::steady_clock::now();
-  if (!b->tuned) LAPH_TRY(autotune_block_width(b));
-  const bool warm = b->o.warm_start && b->have_start;
-  b->have_start = false;
-  b->solved = false;
-  LAPH_HIP(hipStreamSynchronize(b->st));
+// Every lane's solve on its own host thread.
+static int solve_lanes(laph_batch* b, bool warm) {
   const int dev = b->p.device;
   std::vector<std::thread> th;
   for (auto& ln : b->lanes)
@@ -1463,12 +1492,50 @@ static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
         ln.msg = "lane thread: hipSetDevice failed";
         return;
       }
+      const auto l0 = std::chrono::steady_clock::now();
       ln.rc = guarded([&] { return lane_solve(b, &ln, warm); });
+      ln.s_solve = std::chrono::duration<double>(std::chrono::steady_clock::now() - l0).count();
       if (ln.rc != LAPH_OK) ln.msg = laph_last_error();
     });
   for (auto& t : th) t.join();
   for (auto& ln : b->lanes)
     if (ln.rc != LAPH_OK) return set_error(ln.rc, "%s", ln.msg.c_str());
+  return LAPH_OK;
+}
+
+static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
+  auto t0 = std::chrono::steady_clock::now();
+  if (!b->tuned) LAPH_TRY(autotune_block_width(b));
+  const bool warm = b->o.warm_start && b->have_start;
+  b->have_start = false;
+  b->solved = false;
+  LAPH_HIP(hipStreamSynchronize(b->st));
+  const std::vector<int> wc = b->warm_tuned || warm ? std::vector<int>() : warm_candidates(b);
+  if (wc.empty()) {
+    LAPH_TRY(solve_lanes(b, warm));
+  } else {
+    // first solve from the start block: one whole solve per warm-up degree
+    // (each a valid solve), timed without the host time of first-use graph
+    // captures; the fastest is kept, and solved again when it was not the last
+    // one run, so that this solve's pairs are those every later solve repeats
+    double best = 0.0;
+    int dbest = wc.back();
+    for (size_t i = 0; i < wc.size(); ++i) {
+      b->warm_deg = wc[i];
+      LAPH_TRY(solve_lanes(b, false));
+      double sec = 0.0;
+      for (auto& ln : b->lanes) sec = std::max(sec, ln.s_solve - ln.s_capture);
+      if (b->trace) std::fprintf(stderr, "[laph] warm-up degree %d: %.3f s\n", wc[i], sec);
+      if (i == 0 || sec < best) best = sec, dbest = wc[i];
+    }
+    b->warm_deg = dbest;
+    if (dbest != wc.back()) LAPH_TRY(solve_lanes(b, false));
+  }
+  if (!warm) b->warm_tuned = true;
+  if (!b->tune_miss_key.empty() && b->warm_tuned) {
+    if (const char* tf = std::getenv("LAPH_TUNE_FILE")) tune_store(b, tf, b->tune_miss_key);
+    b->tune_miss_key.clear();
+  }
   // fix_phase (eigensystem.cpp:4-30) of every timeslice's nev eigenvectors,
   // in place in B[0]: the solve ends with phase-fixed eigenpairs resident
   LAPH_HIP(launch_fix_phases(b->B[0], b->phase, b->n, b->K, b->nev, b->nts, b->st));
@@ -1504,6 +1571,7 @@ static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
   s.filter_strip = b->strip;
   s.filter_zrun = b->bf16 == 0 && (b->bw == 128 || b->bw == 64) ? laph::stencil_zrun(b) : 0;
   s.start_window = b->window0;
+  s.warmup_degree = b->warm_deg > 0 && b->fdeg_max >= b->warm_deg ? b->warm_deg : b->deg;
   s.converged = conv;
   s.max_residual = maxres;
   s.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1573,7 +1641,10 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     b->per_ts = (size_t)n * K;
     b->trace = std::getenv("LAPH_TRACE") != nullptr;
     if (const char* e = std::getenv("LAPH_WARM")) b->warm_cycles = std::max(0, std::atoi(e));
-    if (const char* e = std::getenv("LAPH_WARM_DEG")) b->warm_deg = std::max(0, std::atoi(e)) / 4 * 4;
+    if (const char* e = std::getenv("LAPH_WARM_DEG")) {
+      b->warm_deg = std::max(0, std::atoi(e)) / 4 * 4;
+      b->warm_deg_fixed = true;
+    }
     b->force_shift = std::getenv("LAPH_SHIFTED_CHOLQR") != nullptr;
     if (const char* e = std::getenv("LAPH_QR_CYCLES")) b->qr_cycles = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("LAPH_TEST_POISON")) b->poison_ts = std::atoi(e);

@@ -49,6 +49,7 @@ struct laph_lane {
   int min_pivot_ts = -1;  // the timeslice it belongs to
   double fbytes = 0;
   float ms_filter = 0, ms_dense = 0;
+  double s_solve = 0, s_capture = 0;  // wall time of the lane's solve, host time capturing filter graphs
 };
 
 struct laph_batch {
@@ -105,6 +106,9 @@ struct laph_batch {
   bool warmup_cap = true;  // warm-up cycle after a Rayleigh-Ritz start: window at most window0
                           // (LAPH_WARMUP_WINDOW_CAP=0: from the start block's Ritz values)
   int warm_deg = 0;     // degree of the warm-up cycles in the per-cycle degree mode (0: the batch's degree)
+  bool warm_deg_fixed = false;  // LAPH_WARM_DEG set: the warm-up degree is not timed
+  bool warm_tuned = false;      // the warm-up degree was timed (or read from the layout cache)
+  std::string tune_miss_key;    // layout cache key to append once the warm-up degree is chosen
   int strip = 0;  // sweep strip width of the stencil kernels (divides L; L = x-major planes)
   bool strip_fixed = false;  // LAPH_STRIP given
   int zrun = 0;  // z-sites per wavefront of the y-row work-group filter step (LAPH_ZRUN; 0: one site each)

@@ -440,10 +440,10 @@ def test_layout_cache(lp, gpu, tmp_path, monkeypatch):
     assert len(lines) == 1 and " : bw " in lines[0]
     key, val = lines[0].split(" : ")
     # an invalid entry for the same key (a width that does not divide K) comes first and is skipped
-    cache.write_text(f"{key} : bw 48 strip 3 split 1 chunks 1 mfma 0\n{lines[0]}\n")
+    cache.write_text(f"{key} : bw 48 strip 3 split 1 chunks 1 mfma 0 warm {deg}\n{lines[0]}\n")
     st2, ev2 = run()
     assert st2.layout_from_cache == 1
-    for f in ("filter_block", "filter_strip", "gram_blocks", "gram_chunks", "gram_mfma"):
+    for f in ("filter_block", "filter_strip", "gram_blocks", "gram_chunks", "gram_mfma", "warmup_degree"):
         assert getattr(st2, f) == getattr(st1, f), f
     for a, b in zip(ev1, ev2):
         assert np.array_equal(a, b)
@@ -487,3 +487,39 @@ def test_rayleigh_ritz_start_keeps_the_start_window(lp, gpu, monkeypatch):
         evp = lp.recover_spectrum(theta, deg, 12.0, lamC)
         for name in runs:
             np.testing.assert_allclose(runs[name][2][t], evp, rtol=1e-10, atol=1e-12, err_msg=name)
+
+
+def test_warmup_degree_timed_on_first_solve(lp, gpu, monkeypatch):
+    """The first solve from the start block solves the batch once per
+    warm-up degree candidate (DEG, then 24, 32 and 40 above it within the
+    amplification bound at the start window) and keeps the fastest; its pairs
+    are those of the chosen degree, so a second solve of the batch repeats
+    them bit for bit.  LAPH_WARM_DEG fixes the degree and skips the timing."""
+    L, nts, nev, deg, tol = 12, 2, 40, 8, 1e-10
+    data = smeared(L, 41, 0.4, nts)
+
+    def batch():
+        b = lp.Batch(L, deg, 12.0, 0.0, nts, nev, nsub=64, tol=tol, seed=7, device=gpu)
+        for t, (ts, _) in enumerate(data):
+            b.set_timeslice(t, ts, 0.62, 0.62, 3)
+        return b
+
+    b = batch()
+    st1 = b.solve()
+    ev1 = [b.get(t, want_vectors=False)[0] for t in range(nts)]
+    st2 = b.solve()
+    ev2 = [b.get(t, want_vectors=False)[0] for t in range(nts)]
+    b.close()
+    print(f"warm-up degree {st1.warmup_degree}: first solve {st1.seconds:.2f}s, second {st2.seconds:.2f}s")
+    assert st1.converged == st2.converged == nts and st1.warmup_degree == st2.warmup_degree
+    assert st1.warmup_degree in (deg, 24, 32, 40) or (deg < st1.warmup_degree < 40 and st1.warmup_degree % 4 == 0)
+    for a, c in zip(ev1, ev2):
+        assert np.array_equal(a, c)
+    monkeypatch.setenv("LAPH_WARM_DEG", "24")
+    b = batch()
+    st3 = b.solve()
+    ev3 = [b.get(t, want_vectors=False)[0] for t in range(nts)]
+    b.close()
+    assert st3.warmup_degree == 24 and st3.converged == nts
+    for a, c in zip(ev1, ev3):
+        np.testing.assert_allclose(c, a, rtol=1e-10, atol=1e-12)
