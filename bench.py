@@ -51,8 +51,8 @@ PEAK_FP64_MFMA = 78.6e12  # dense FP64 matrix peak of the MI355X
 # 1 GiB device copy (python bench.py --summarize-pmc ...), with the sha256
 # (first 16 hex digits) of the source file of the kernel it was measured on:
 # the figure is reported only while that file is unchanged.
-TRAFFIC = {"bytes_per_launch": 43388996310, "source": "laph_eigsys_amd/csrc/kernels_block.hip",
-           "sha16": "547fc18aa6081205"}  # profiles/filter_step_32x32x32x64_k256_pmc.md
+TRAFFIC = {"bytes_per_launch": 43441229619, "source": "laph_eigsys_amd/csrc/kernels_block.hip",
+           "sha16": "dd16b5a5b4ebc6a2"}  # profiles/filter_step_32x32x32x64_k256_pmc.md
 
 
 def main():
@@ -284,8 +284,11 @@ def profile_filter(lp, args):
     b = lp.Batch(L, DEG, LAMBDA_L, 0.0, nts, NEV, nsub=NSUB, tol=TOL, seed=SEED)
     for t in range(nts):
         b.set_timeslice(t, cfg[t * vol:(t + 1) * vol], 0.62, 0.62, 3)
+    t0 = time.perf_counter()
     b.solve()
+    print(f"[bench] profile-filter: solve {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     ms, by = b.time_filter(reps=args.filter_reps)
+    print(f"[bench] profile-filter: {args.filter_reps} replays, {ms:.3f} ms per launch", file=sys.stderr, flush=True)
     b.close()
     hip = ctypes.CDLL("libamdhip64.so")
     nbytes = 1 << 30
