@@ -23,9 +23,10 @@ Solvers:
            damped interval adapts per timeslice, LAM_C is not used); the
            group size fits the work buffers into --mem-gb of HBM, so 48^3 x 96
            streams through in groups of 24.  The first solve of a process
-           times the filter and Gram layouts (several seconds at 32^3 x 64);
-           --tune-file keeps them in a small text file (LAPH_TUNE_FILE) so
-           that the next configurations of a production run read them.
+           times the filter and Gram layouts and the warm-up degree (several
+           seconds and up to four solves at 32^3 x 64); --tune-file keeps the
+           choice in a small text file (LAPH_TUNE_FILE) so that the next
+           configurations of a production run read it.
 
 The reference divides LT over MPI ranks (ev_ts.cpp:96-126); one process
 here takes [tstart, tend] (default: all of LT), the share one rank would get.
