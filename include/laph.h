@@ -198,8 +198,12 @@ typedef struct laph_batch laph_batch;
  *                           applied to r (2); the same with the correction pass reading r in every
  *                           step (1); the three-term recurrence, which also reads the previous vector (0)
  *   LAPH_WARM=n             cycles of T_DEG(B) before the per-cycle degree (or T_m(B)^P) cycles (1)
- *   LAPH_WARM_DEG=d         degree of the warm-up cycles in the per-cycle degree mode (DEG; a multiple
- *                           of 4 up to LAPH_FILTER_DEGREE)
+ *   LAPH_WARM_DEG=d         degree of the warm-up cycles in the per-cycle degree mode (a multiple of 4
+ *                           up to LAPH_FILTER_DEGREE; else DEG or a higher degree timed on the first
+ *                           solve: see laph_batch_stats.warmup_degree)
+ *   LAPH_WARMUP_WINDOW_CAP=0  the warm-up cycle after a Rayleigh-Ritz start (a network proposal,
+ *                           LAPH_START_RR) takes its window from the start block's Ritz values (else
+ *                           its lower end is at most the start window)
  *   LAPH_WINDOW0=x          window lower end of the first cycle from the start block (3 x the K-th
  *                           level of the free Laplacian, at most 0.75 lambda_l)
  *   LAPH_QR_CYCLES=n        cycles (and start block) orthonormalised by explicit CholQR2 (0)
@@ -220,10 +224,11 @@ typedef struct laph_batch laph_batch;
  *                           sums (else both timed on the first solve, with the kernel)
  *   LAPH_GRAM_MFMA=0|1      Gram products on rocBLAS zgemm (0) or on the hand-written FP64 MFMA
  *                           kernel (1); else timed on the first solve
- *   LAPH_TUNE_FILE=path     layout cache: the filter and Gram layouts of a shape (device, L, K,
- *                           timeslices) are read from this text file instead of timed on the first
- *                           solve, and appended to it after timing (not used while LAPH_BW,
- *                           LAPH_STRIP, LAPH_GRAM_SPLIT or LAPH_GRAM_MFMA is set)
+ *   LAPH_TUNE_FILE=path     layout cache: the filter and Gram layouts and the warm-up degree of a shape
+ *                           (device, L, K, timeslices, DEG, nev, tol) are read from this text file
+ *                           instead of timed on the first solve, and appended to it after timing (not
+ *                           used while LAPH_BW, LAPH_STRIP, LAPH_GRAM_SPLIT, LAPH_GRAM_MFMA or
+ *                           LAPH_WARM_DEG is set)
  *   LAPH_TRACE              (set) per-cycle trace of the first timeslice on stderr
  *   LAPH_NET_TARGET=1       laph_net_train_step fits the span of all K Ritz vectors of the last solve
  *                           (else its nev eigenvectors)
