@@ -215,6 +215,9 @@ typedef struct laph_batch laph_batch;
  *   LAPH_DEGREE_SAFETY=S    a cycle's degree is lowered to the least d with T_d(theta_j) >= S r_j / tol
  *                           for every unconverged wanted pair (theta_j, r_j) of the lane (30); 0: always the
  *                           highest degree the amplification bound allows
+ *   LAPH_BLOCK_DEGREE=0     one degree per cycle for every 64-column block (else each block is filtered
+ *                           to the degree its own unconverged wanted pairs need by the same rule, at most
+ *                           the cycle's degree)
  *   LAPH_SHIFTED_CHOLQR=1   every CholQR pass starts with a shifted one
  *   LAPH_TRSM=1             Rayleigh-Ritz through triangular solves with the Gram's Cholesky factor
  *                           (else its explicit inverse and GEMMs)

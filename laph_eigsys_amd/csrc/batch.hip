@@ -98,6 +98,7 @@ static int batch_alloc(laph_batch* b) {
   LAPH_HIP(hipMalloc(&b->coef2, (size_t)(b->deg + LAPH_MAX_PASSES - 2) * nk * sizeof(float4)));
   LAPH_HIP(hipMalloc(&b->cab, b->nts * sizeof(float2)));
   LAPH_HIP(hipMalloc(&b->first_blk, b->nts * sizeof(int)));
+  LAPH_HIP(hipMalloc(&b->first_blk_steps, (size_t)csteps * b->nts * sizeof(int)));
   LAPH_HIP(hipMalloc(&b->lock_blk, b->nts * sizeof(int)));
   LAPH_HIP(hipMalloc(&b->cn, nk * sizeof(double)));
   LAPH_HIP(hipMalloc(&b->red, (size_t)red_chunks(b->n) * nk * sizeof(double)));
@@ -108,6 +109,7 @@ static int batch_alloc(laph_batch* b) {
   LAPH_HIP(hipMemset(b->coef2, 0, (size_t)(b->deg + LAPH_MAX_PASSES - 2) * nk * sizeof(float4)));
   LAPH_HIP(hipMemset(b->cab, 0, b->nts * sizeof(float2)));
   LAPH_HIP(hipMemset(b->first_blk, 0, b->nts * sizeof(int)));
+  LAPH_HIP(hipMemset(b->first_blk_steps, 0, (size_t)csteps * b->nts * sizeof(int)));
   LAPH_HIP(hipMemset(b->R32, 0, nb * b->esz));
   for (int i = 0; i < 3; ++i) LAPH_HIP(hipMemset(b->P[i], 0, nb * b->esz));
   const std::vector<int> cand = block_widths(b);
@@ -160,7 +162,8 @@ static void batch_free(laph_batch* b) {
     if (q) (void)hipFree(q);
   void* bufs[] = {b->U64, b->U32, b->U32c, b->dec,  b->Unew, b->B[0], b->B[1],   b->B[2], b->R32,     b->P[0],
                   b->P[1], b->P[2], b->G,  b->G2,   b->P2,   b->G3,   b->mu,     b->E,    b->info,    b->nrm2,
-                  b->gam, b->coef, b->cab, b->first_blk, b->lock_blk, b->cn, b->coef2, b->jsweeps,
+                  b->gam, b->coef, b->cab, b->first_blk, b->first_blk_steps, b->lock_blk, b->cn, b->coef2,
+                  b->jsweeps,
                   b->red, b->phase, b->Gpart};
   for (void* q : bufs)
     if (q) (void)hipFree(q);
@@ -782,6 +785,42 @@ static void correction_q_coefficients(const std::vector<std::pair<double, double
   }
 }
 
+// Filter degree of each 64-column block of lane timeslice t in a cycle of
+// degree m (per-cycle degree mode): m, lowered to what the block's unconverged
+// wanted columns need by the margin rule of cycle_degree (a multiple of 4, at
+// least 4; a block whose wanted columns have all converged takes 4), kept at m
+// for blocks of guard columns only and where a wanted Ritz value sits at the
+// window edge, and made non-decreasing over the blocks.  The lower blocks hold
+// the lower eigenvalues, which converge first, so the blocks still filtered at
+// step s are a suffix: the filter kernels' first-active-block argument of that
+// step, with no change to the kernels.  A block of degree d < m ends its
+// d - 1 steps in the buffer the pass ends in (d and m are both multiples of 4).
+static void block_degrees(const laph_batch* b, const laph_lane* ln, int t, double lct, int m, int* d) {
+  const int nb = b->K / 64, T = ln->t0 + t;
+  const double c = 2.0 / (b->p.lambda_l - lct);
+  const double* mu = &b->h_mu[(size_t)T * b->K];
+  const double* res = &b->h_res[(size_t)T * b->K];
+  int prev = 4;
+  for (int blk = 0; blk < nb; ++blk) {
+    int db = m;
+    if (blk * 64 < b->nev) {
+      double need = 0.0;
+      bool est = true;
+      for (int j = std::max(blk * 64, b->nlock[T]); j < std::min((blk + 1) * 64, b->nev) && est; ++j) {
+        const double r = std::sqrt(res[j]);
+        if (r <= b->o.tol) continue;
+        const double th = 1.0 + c * (lct - mu[j]);
+        if (!(th > 1.0 + 1e-12)) est = false;
+        else need = std::max(need, std::acosh(b->deg_safety * r / b->o.tol) / std::acosh(th));
+      }
+      if (est) db = std::min(m, std::max(4, (int)std::ceil(need / 4.0) * 4));
+    }
+    db = std::max(db, prev);
+    prev = db;
+    d[blk] = db;
+  }
+}
+
 static int upload_coefficients(const View& v, const std::vector<double>& lc, int passes) {
   laph_batch* b = v.b;
   laph_lane* ln = v.ln;
@@ -792,6 +831,10 @@ static int upload_coefficients(const View& v, const std::vector<double>& lc, int
   std::vector<double> ga((size_t)nt * K);
   std::vector<float2> cab(nt);
   std::vector<int> fb(nt);
+  const int nb64 = K / 64;
+  ln->dblk.assign((size_t)nt * nb64, m);
+  // per-cycle degree mode: first filtered 64-column block of every correction step
+  std::vector<int> fbs(b->fdeg_max > 0 ? (size_t)(m - 1) * nt : 0);
   // the timeslices' tables are independent: filled by a few host threads
   auto fill = [&](int tb, int te) {
     std::vector<double> tau(m + 1), s(m + 1);
@@ -801,13 +844,23 @@ static int upload_coefficients(const View& v, const std::vector<double>& lc, int
       cab[t] = make_float2((float)c, (float)(a - 6.0 * c));
       const int nl = b->nlock[ln->t0 + t];
       fb[t] = nl >= b->nev ? K / 64 : nl / 64;
+      int* dt = &ln->dblk[(size_t)t * nb64];
+      if (b->fdeg_max > 0) {
+        if (ln->blk_cycle) block_degrees(b, ln, t, lc[t], m, dt);
+        for (int st = 1; st < m; ++st) {  // blocks of degree d are filtered in steps 1 .. d-1
+          int f = fb[t];
+          while (f < nb64 && dt[f] - 1 < st) ++f;
+          fbs[(size_t)(st - 1) * nt + t] = f;
+        }
+      }
       const double* mu = &b->h_mu[v.nk() + (size_t)t * K];
       for (int j = 0; j < K; ++j) {
         const double th = 1.0 + c * (lc[t] - mu[j]);  // Ritz value in B
-        if (b->fdeg_max > 0) {  // one correction pass of this cycle's degree
-          const double tm = std::real(std::cos((double)m * std::acos(std::complex<double>(th, 0.0))));
+        if (b->fdeg_max > 0) {  // one correction pass of its block's degree (this cycle's degree or lower)
+          const int d = dt[j / 64];
+          const double tm = std::real(std::cos((double)d * std::acos(std::complex<double>(th, 0.0))));
           const double sm = std::max(std::fabs(tm), 1.0);
-          correction_q_coefficients(b->qrot[m], th, m, sm, &co[(size_t)t * K + j], (size_t)nt * K);
+          correction_q_coefficients(b->qrot[d], th, d, sm, &co[(size_t)t * K + j], (size_t)nt * K);
           ga[(size_t)t * K + j] = tm / sm;
           continue;
         }
@@ -885,6 +938,9 @@ static int upload_coefficients(const View& v, const std::vector<double>& lc, int
   LAPH_HIP(hipMemcpyAsync(b->gam + v.nk(), ga.data(), ga.size() * sizeof(double), hipMemcpyHostToDevice, ln->st));
   LAPH_HIP(hipMemcpyAsync(b->cab + ln->t0, cab.data(), cab.size() * sizeof(float2), hipMemcpyHostToDevice, ln->st));
   LAPH_HIP(hipMemcpyAsync(b->first_blk + ln->t0, fb.data(), fb.size() * sizeof(int), hipMemcpyHostToDevice, ln->st));
+  if (!fbs.empty())
+    LAPH_HIP(hipMemcpy2DAsync(b->first_blk_steps + ln->t0, (size_t)b->nts * sizeof(int), fbs.data(), nt * sizeof(int),
+                              nt * sizeof(int), m - 1, hipMemcpyHostToDevice, ln->st));
   return LAPH_OK;
 }
 
@@ -900,10 +956,11 @@ static int enqueue_filter(const View& v) {
     const void* in = st == 1 ? R : fb(b, b->P[(st - 2) % nb], vo);
     const void* old = st >= 3 && !b->prod ? fb(b, b->P[(st - 3) % 3], vo) : nullptr;
     void* out = fb(b, b->P[(st - 1) % nb], vo);
+    // per-cycle degree mode: the step's own first filtered block (block degrees)
+    const int* fbl = b->fdeg_max > 0 ? b->first_blk_steps + (size_t)(st - 1) * b->nts : b->first_blk;
     LAPH_HIP(launch_filter_step(b->U32c + v.lo(), in, b->qform ? nullptr : R, old, out,
-                                b->coef + (st - 1) * step + v.nk(),
-                                b->cab + v.ln->t0, b->first_blk + v.ln->t0, L, K, nt, b->bw, b->bf16, b->strip,
-                                stencil_zrun(b), v.ln->st));
+                                b->coef + (st - 1) * step + v.nk(), b->cab + v.ln->t0, fbl + v.ln->t0, L, K, nt,
+                                b->bw, b->bf16, b->strip, stencil_zrun(b), v.ln->st));
   }
   return LAPH_OK;
 }
@@ -1173,10 +1230,25 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
     ln->m = b->fdeg_max > 0 && !warmup ? cycle_degree(b, ln, lc) : (warmup && wdeg ? b->warm_deg : b->deg);
     const int passes = b->fdeg_max > 0 || warmup ? 1 : filter_passes(b, lc);
     const int m = ln->m;
+    ln->blk_cycle = b->block_degrees && b->fdeg_max > 0 && !warmup && b->deg_safety > 0.0;
     LAPH_TRY(upload_coefficients(v, lc, passes));
     for (int t = 0; t < nt; ++t) {
       const int nl = b->nlock[ln->t0 + t];
       if (nl >= b->nev) continue;
+      if (b->fdeg_max > 0) {  // correction pass of block degrees: step s filters the blocks of degree > s
+        const int nb64 = K / 64;
+        const int* dt = &ln->dblk[(size_t)t * nb64];
+        for (int st = 1; st < m; ++st) {
+          int f = nl / 64;
+          while (f < nb64 && dt[f] - 1 < st) ++f;
+          const int cols = K - f * 64;
+          if (cols == 0) break;
+          const int nblk = (cols + b->bw - 1) / b->bw;
+          ln->sv32 += cols;
+          ln->fbytes += (double)b->V3 * ((double)cols * 2 * 3.0 * b->esz + nblk * 216.0);
+        }
+        continue;
+      }
       const int cols = K - (nl / 64) * 64;
       const int nblk = (cols + b->bw - 1) / b->bw;  // blocks the kernel visits (links once each)
       ln->sv32 += (long long)(m - 1) * cols;
@@ -1237,7 +1309,9 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
       // per 64-column block of timeslice 0: largest residual of its unconverged wanted columns, and
       // acosh(theta) (theta in the next cycle's window) of its slowest wanted column
       const double c = 2.0 / (b->p.lambda_l - lc[0]);
-      std::fprintf(stderr, "[laph]   blocks:");
+      std::fprintf(stderr, "[laph]   block degrees:");
+      for (int blk = 0; blk < K / 64; ++blk) std::fprintf(stderr, " %d", ln->dblk[blk]);
+      std::fprintf(stderr, "\n[laph]   blocks:");
       for (int blk = 0; blk * 64 < b->nev; ++blk) {
         double rmax = 0.0;
         const int jend = std::min((blk + 1) * 64, b->nev);
@@ -1671,6 +1745,7 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
       if (w > 0.0) b->window0 = std::min(w, 0.75 * p->lambda_l);
     }
     if (const char* e = std::getenv("LAPH_WARMUP_WINDOW_CAP")) b->warmup_cap = std::atoi(e) != 0;
+    if (const char* e = std::getenv("LAPH_BLOCK_DEGREE")) b->block_degrees = std::atoi(e) != 0;
     b->roots = laph::leja_roots(p->deg);
     b->strip = strip > 0 ? strip : p->L;
     b->strip_fixed = strip > 0;
@@ -1764,9 +1839,12 @@ int laph_batch_time_filter(laph_batch* b, int reps, double* ms_per_launch, doubl
     ln->m = 0;  // the batch's degree
     const laph::View v{b, ln};
     const int K = b->K, nt = ln->nt;
-    // every column of every timeslice of lane 0 active
+    // every column of every timeslice of lane 0 active (also in every step of the correction pass)
     std::vector<int> zero(nt, 0);
     LAPH_HIP(hipMemcpyAsync(b->first_blk + ln->t0, zero.data(), nt * sizeof(int), hipMemcpyHostToDevice, ln->st));
+    for (int s = 1; b->fdeg_max > 0 && s < b->deg; ++s)
+      LAPH_HIP(hipMemcpyAsync(b->first_blk_steps + (size_t)(s - 1) * b->nts + ln->t0, zero.data(), nt * sizeof(int),
+                              hipMemcpyHostToDevice, ln->st));
     void* w = nullptr;
     LAPH_TRY(laph::run_filter(v, 1, &w));  // warm-up (and capture if needed)
     LAPH_HIP(hipEventRecord(ln->ev[0], ln->st));

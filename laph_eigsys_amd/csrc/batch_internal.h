@@ -33,6 +33,8 @@ struct laph_lane {
   hipGraphExec_t graph = nullptr;  // correction-form filter pass of the batch's degree
   std::map<int, hipGraphExec_t> gdeg;  // correction passes by degree (per-cycle degree mode)
   int m = 0;                    // filter degree of the current cycle (0: the batch's degree)
+  bool blk_cycle = false;       // this cycle's correction pass uses per-block degrees
+  std::vector<int> dblk;        // [lane timeslice][K/64] filter degree of each 64-column block this cycle
   double wcap = 0.0;            // cap of the window's lower end: 0.75 lambda_l, or the start window
                                 // in a warm-up cycle that follows a Rayleigh-Ritz start
   hipGraphExec_t gplain[LAPH_MAX_PASSES - 1] = {};  // plain passes 2 .. LAPH_MAX_PASSES
@@ -87,6 +89,8 @@ struct laph_batch {
   float4* coef2 = nullptr;   // [deg - 1 + LAPH_MAX_PASSES - 1][nts][K] plain passes
   float2* cab = nullptr;     // [nts]
   int* first_blk = nullptr;  // [nts] first filtered 64-column block
+  int* first_blk_steps = nullptr;  // [correction step][nts] first filtered 64-column block of each step of
+                                   // the per-cycle degree mode's correction pass (block degrees)
   int* lock_blk = nullptr;   // [nts] first active 64-column block of the dense steps
   // host mirrors, written by the lanes on disjoint ranges
   std::vector<int> nlock;
@@ -108,6 +112,7 @@ struct laph_batch {
   int warm_deg = 0;     // degree of the warm-up cycles in the per-cycle degree mode (0: the batch's degree)
   bool warm_deg_fixed = false;  // LAPH_WARM_DEG set: the warm-up degree is not timed
   bool warm_tuned = false;      // the warm-up degree was timed (or read from the layout cache)
+  bool block_degrees = true;    // per-block degrees in the per-cycle degree mode (LAPH_BLOCK_DEGREE=0: off)
   std::string tune_miss_key;    // layout cache key to append once the warm-up degree is chosen
   int strip = 0;  // sweep strip width of the stencil kernels (divides L; L = x-major planes)
   bool strip_fixed = false;  // LAPH_STRIP given

@@ -523,3 +523,34 @@ def test_warmup_degree_timed_on_first_solve(lp, gpu, monkeypatch):
     assert st3.warmup_degree == 24 and st3.converged == nts
     for a, c in zip(ev1, ev3):
         np.testing.assert_allclose(c, a, rtol=1e-10, atol=1e-12)
+
+
+def test_block_degrees(lp, gpu, monkeypatch):
+    """Per-block filter degrees (each 64-column block filtered only to the
+    degree its unconverged wanted columns need, LAPH_BLOCK_DEGREE=1, the
+    default) against one degree per cycle (LAPH_BLOCK_DEGREE=0): the same
+    spectrum to 1e-10 as the plain FP64 drop-in, every residual <= tol, with
+    no more FP32 stencil work."""
+    L, nts, nev, deg, tol = 16, 3, 100, 8, 1e-10
+    data = smeared(L, 97, 0.35, nts)
+    monkeypatch.setenv("LAPH_WARM_DEG", str(deg))
+    runs = {}
+    for on in (0, 1):
+        monkeypatch.setenv("LAPH_BLOCK_DEGREE", str(on))
+        b = lp.Batch(L, deg, 12.0, 0.0, nts, nev, nsub=128, tol=tol, seed=12, device=gpu)
+        for t, (ts, _) in enumerate(data):
+            b.set_timeslice(t, ts, 0.62, 0.62, 3)
+        st = b.solve()
+        assert st.converged == nts and st.max_residual <= tol
+        runs[on] = (st.iterations, st.stencil_vecs, [b.get(t, want_vectors=False)[0] for t in range(nts)])
+        b.close()
+    print({k: v[:2] for k, v in runs.items()})  # (cycles, FP32 stencil applications)
+    assert runs[1][1] <= runs[0][1]
+    for t, (ts, _) in enumerate(data):
+        lamC = -runs[0][2][t][-1] * 1.3
+        with lp.Tslice(L, deg, 12.0, lamC, device=gpu, ts_config=ts) as tsl:
+            tsl.smearing_hyp(0.62, 0.62, 3)
+            theta, _, _ = tsl.eps_solve(nev, tol=1e-12, want_vectors=False)
+        evp = lp.recover_spectrum(theta, deg, 12.0, lamC)
+        for on in (0, 1):
+            np.testing.assert_allclose(runs[on][2][t], evp, rtol=1e-10, atol=1e-12, err_msg=str(on))
