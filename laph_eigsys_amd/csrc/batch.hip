@@ -1303,9 +1303,9 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
       std::fprintf(stderr,
                    "[laph] cycle %d: degree %d passes %d  window %.4f  mu[nev-1] %.5f  mu[K-1] %.4f  res[0] %.2e  "
                    "res[nev-1] %.2e  worst %.2e  locked %d  lock %d  min pivot^2 %.2e (timeslice %d)  rr fallbacks %d  "
-                   "shifted %d\n", it, m, passes, lc[0], mu[b->nev - 1], mu[K - 1], std::sqrt(rs[0]),
-                   std::sqrt(rs[b->nev - 1]), worst, b->nlock[0], ln->lock, ln->min_pivot2, ln->min_pivot_ts,
-                   ln->rr_fallbacks, ln->shifted);
+                   "shifted %d  unconverged timeslices %d of %d\n", it, m, passes, lc[0], mu[b->nev - 1], mu[K - 1],
+                   std::sqrt(rs[0]), std::sqrt(rs[b->nev - 1]), worst, b->nlock[0], ln->lock, ln->min_pivot2,
+                   ln->min_pivot_ts, ln->rr_fallbacks, ln->shifted, active, nt);
       // per 64-column block of timeslice 0: largest residual of its unconverged wanted columns, and
       // acosh(theta) (theta in the next cycle's window) of its slowest wanted column
       const double c = 2.0 / (b->p.lambda_l - lc[0]);
