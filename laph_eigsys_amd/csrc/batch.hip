@@ -834,7 +834,8 @@ static int upload_coefficients(const View& v, const std::vector<double>& lc, int
   const int nb64 = K / 64;
   ln->dblk.assign((size_t)nt * nb64, m);
   // per-cycle degree mode: first filtered 64-column block of every correction step
-  std::vector<int> fbs(b->fdeg_max > 0 ? (size_t)(m - 1) * nt : 0);
+  std::vector<int>& fbs = ln->fbs;
+  fbs.assign(b->fdeg_max > 0 ? (size_t)(m - 1) * nt : 0, 0);
   // the timeslices' tables are independent: filled by a few host threads
   auto fill = [&](int tb, int te) {
     std::vector<double> tau(m + 1), s(m + 1);
@@ -938,9 +939,9 @@ static int upload_coefficients(const View& v, const std::vector<double>& lc, int
   LAPH_HIP(hipMemcpyAsync(b->gam + v.nk(), ga.data(), ga.size() * sizeof(double), hipMemcpyHostToDevice, ln->st));
   LAPH_HIP(hipMemcpyAsync(b->cab + ln->t0, cab.data(), cab.size() * sizeof(float2), hipMemcpyHostToDevice, ln->st));
   LAPH_HIP(hipMemcpyAsync(b->first_blk + ln->t0, fb.data(), fb.size() * sizeof(int), hipMemcpyHostToDevice, ln->st));
-  if (!fbs.empty())
-    LAPH_HIP(hipMemcpy2DAsync(b->first_blk_steps + ln->t0, (size_t)b->nts * sizeof(int), fbs.data(), nt * sizeof(int),
-                              nt * sizeof(int), m - 1, hipMemcpyHostToDevice, ln->st));
+  for (int st = 1; !fbs.empty() && st < m; ++st)  // one row per step, as the coefficient table
+    LAPH_HIP(hipMemcpyAsync(b->first_blk_steps + (size_t)(st - 1) * b->nts + ln->t0, fbs.data() + (size_t)(st - 1) * nt,
+                            nt * sizeof(int), hipMemcpyHostToDevice, ln->st));
   return LAPH_OK;
 }
 

@@ -35,6 +35,8 @@ struct laph_lane {
   int m = 0;                    // filter degree of the current cycle (0: the batch's degree)
   bool blk_cycle = false;       // this cycle's correction pass uses per-block degrees
   std::vector<int> dblk;        // [lane timeslice][K/64] filter degree of each 64-column block this cycle
+  std::vector<int> fbs;         // [correction step][lane timeslice] host copy of the step table: kept
+                                // here so that it outlives the asynchronous uploads of the cycle
   double wcap = 0.0;            // cap of the window's lower end: 0.75 lambda_l, or the start window
                                 // in a warm-up cycle that follows a Rayleigh-Ritz start
   hipGraphExec_t gplain[LAPH_MAX_PASSES - 1] = {};  // plain passes 2 .. LAPH_MAX_PASSES
