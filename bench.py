@@ -62,6 +62,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nts", type=int, default=LT)
     ap.add_argument("--plain-ts", type=int, default=LT, help="timeslices solved by the plain FP64 drop-in")
+    ap.add_argument("--batch-tol", type=float, default=TOL,
+                    help="bound on ||D v - lambda v|| for the batched solver (the plain drop-in keeps the "
+                         "reference's -eps_tol 1e-10 on T_DEG(B)); 5.8e-12 meets that criterion for every "
+                         "wanted pair of the headline field")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-full", action="store_true",
                     help="only the CPU baseline as planned: one timeslice per host core, each solved to 1e-10 by "
@@ -137,7 +141,8 @@ def main():
     t0 = time.time()
     cfg = lp.synth_config(SEED, EPS, L, nts)
     vol = 72 * L ** 3
-    b = lp.Batch(L, DEG, LAMBDA_L, 0.0, nts, NEV, nsub=NSUB, tol=TOL, seed=SEED)
+    btol = args.batch_tol
+    b = lp.Batch(L, DEG, LAMBDA_L, 0.0, nts, NEV, nsub=NSUB, tol=btol, seed=SEED)
     t_field = time.time() - t0
     sync = device_sync()
     sync()
@@ -233,12 +238,12 @@ def main():
                                f"one filter pass per cycle of degree <= 72 after one of degree {st.warmup_degree}, "
                                f"vector block "
                                f"{st.filter_block}, sweep strip {st.filter_strip}, search space {NSUB}, "
-                               f"residual <= {TOL}, ending with phase-fixed eigenpairs in HBM",
-                   "lattice": f"{L}^3x{nts}", "nev": NEV, "deg": DEG, "warmup_degree": st.warmup_degree, "tol": TOL,
+                               f"residual <= {btol}, ending with phase-fixed eigenpairs in HBM",
+                   "lattice": f"{L}^3x{nts}", "nev": NEV, "deg": DEG, "warmup_degree": st.warmup_degree, "tol": btol,
                    "parallelism": "none (1 GPU)"},
         "roofline": roofline,
         "gram": gram,
-        "plain_fp64": {"value": plain_value, "unit": "timeslices/s", "timeslices": len(plain_times),
+        "plain_fp64": {"value": plain_value, "unit": "timeslices/s", "timeslices": len(plain_times), "eps_tol": TOL,
                        "lambda_c": lamC, "cycles_max": max(plain_cycles) if plain_cycles else None,
                        "seconds_per_timeslice": spread(plain_times),
                        "max_rel_eigenvalue_difference_vs_batched": plain_dev},
