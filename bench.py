@@ -95,6 +95,9 @@ def main():
     ap.add_argument("--filter-bound", action="store_true",
                     help="only run laph_eigsys_amd/filter_bound: the filter step beside variants with a part taken "
                          "away and copies with its load pattern (32^3, 64 timeslices, 256 vectors)")
+    ap.add_argument("--peak-rates", action="store_true",
+                    help="only run laph_eigsys_amd/peak_rates: FP32 and FP64 FMA and FP64 MFMA issue rates with no "
+                         "memory traffic, at the shader clock measured during the run")
     ap.add_argument("--last", type=int, default=0,
                     help="with --summarize-*: the isolated filter replays are the last LAST filter dispatches")
     args = ap.parse_args()
@@ -112,6 +115,9 @@ def main():
         return
     if args.filter_bound:
         print(json.dumps(run_measure("filter_bound")))
+        return
+    if args.peak_rates:
+        print(json.dumps(run_measure("peak_rates")))
         return
     if args.cpu_worker:
         cpu_worker(args.cpu_worker)

@@ -16,11 +16,6 @@
 //                             full load pattern)
 //   copy_zwalk_lds_x_ycached  the y-edge rows taken from the own line
 //   stream_copy               grid-stride float4 copy (1024 work-groups)
-//   fma_rate                  FP32 FMA issue rate with no memory traffic (8
-//                             independent chains per lane, 8 wavefronts per
-//                             SIMD): FMAs per cycle per SIMD at the shader
-//                             clock measured inside the kernel, which prices
-//                             the step's VALU instructions
 //
 // Prints one JSON object: microseconds per launch (mean of 10 after a warm-up
 // round) and GB/s of the step's algorithmic bytes (2 x 24 B per site and
@@ -291,29 +286,6 @@ __global__ __launch_bounds__(256) void k_stream(const float4* __restrict__ a, fl
   }
 }
 
-// Each lane runs 8 independent FMA chains; the work-group's first lane also
-// records the shader-clock and constant-clock ticks over the loop.
-__global__ __launch_bounds__(256) void k_fma_rate(float* __restrict__ o, long long* __restrict__ clk, float b,
-                                                  float c, int iters) {
-  float a[8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) a[i] = (float)(threadIdx.x + i) * 1e-3f;
-  const long long t0 = clock64(), w0 = wall_clock64();
-  for (int it = 0; it < iters; ++it) {
-#pragma unroll
-    for (int i = 0; i < 8; ++i) a[i] = __builtin_fmaf(a[i], b, c);
-  }
-  const long long t1 = clock64(), w1 = wall_clock64();
-  float s = 0.f;
-#pragma unroll
-  for (int i = 0; i < 8; ++i) s += a[i];
-  o[(size_t)blockIdx.x * 256 + threadIdx.x] = s;
-  if (threadIdx.x == 0) {
-    clk[2 * blockIdx.x] = t1 - t0;
-    clk[2 * blockIdx.x + 1] = w1 - w0;
-  }
-}
-
 __global__ void k_fill(float* p, size_t n, unsigned seed) {
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
     unsigned h = (unsigned)i * 2654435761u ^ seed;
@@ -414,33 +386,6 @@ int main(int argc, char** argv) {
   timeit("copy_zwalk_lds_x_y", [&] { k_zcopy<3><<<grid, 256>>>(in4, o2, L, K, nts, W); });
   timeit("copy_zwalk_lds_x_ycached", [&] { k_zcopy<4><<<grid, 256>>>(in4, o2, L, K, nts, W); });
   timeit("stream_copy", [&] { k_stream<<<1024, 256>>>(in4, o2, nvec / 4); });
-  {
-    int dev = 0, ncu = 0, wall_khz = 0;
-    CK(hipGetDevice(&dev));
-    CK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    CK(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, dev));
-    const int nblk = ncu * 8, iters = 1 << 16;  // 8 wavefronts per SIMD
-    long long* clk;
-    CK(hipMalloc(&clk, (size_t)nblk * 2 * sizeof(long long)));
-    float ms = 0;
-    for (int round = 0; round < 2; ++round) {
-      CK(hipEventRecord(e0));
-      k_fma_rate<<<nblk, 256>>>(out2, clk, 0.999f, 1e-3f, iters);
-      CK(hipGetLastError());
-      CK(hipEventRecord(e1));
-      CK(hipEventSynchronize(e1));
-      CK(hipEventElapsedTime(&ms, e0, e1));
-    }
-    long long c2[2] = {0, 0};
-    CK(hipMemcpy(c2, clk, sizeof c2, hipMemcpyDeviceToHost));
-    CK(hipFree(clk));
-    const double fmas = (double)nblk * 256 * 8 * iters;
-    const double mhz = c2[1] > 0 ? (double)c2[0] / c2[1] * wall_khz * 1e-3 : 0.0;
-    const double per_cycle = mhz > 0 ? fmas / (ms * 1e-3) / (ncu * 4.0 * mhz * 1e6) : 0.0;
-    std::snprintf(buf, sizeof buf, ", \"fma_rate_us\": %.1f, \"fma_rate_TFLOPs\": %.2f, \"fma_rate_clock_MHz\": %.0f, "
-                  "\"fma_rate_per_cycle_per_simd\": %.2f", ms * 1e3, 2.0 * fmas / (ms * 1e-3) / 1e12, mhz, per_cycle);
-    js += buf;
-  }
   js += "}";
   std::printf("%s\n", js.c_str());
   return 0;

@@ -116,9 +116,9 @@ def test_host_info(bench):
 
 def test_measurement_programs_built_and_guarded():
     """The measurement programs bench.py runs (--stream-ceiling,
-    --filter-bound) are built with the library; filter_bound rejects a
-    lattice its sweep cannot tile before it touches a device."""
-    for name in ("stream_ceiling", "filter_bound"):
+    --filter-bound, --peak-rates) are built with the library; filter_bound
+    rejects a lattice its sweep cannot tile before it touches a device."""
+    for name in ("stream_ceiling", "filter_bound", "peak_rates"):
         assert os.access(os.path.join(ROOT, "laph_eigsys_amd", name), os.X_OK), name
     r = subprocess.run([os.path.join(ROOT, "laph_eigsys_amd", "filter_bound"), "12"], capture_output=True, text=True,
                        timeout=60)
