@@ -232,7 +232,8 @@ typedef struct laph_batch laph_batch;
  *                           instead of timed on the first solve, and appended to it after timing (not
  *                           used while LAPH_BW, LAPH_STRIP, LAPH_GRAM_SPLIT, LAPH_GRAM_MFMA or
  *                           LAPH_WARM_DEG is set)
- *   LAPH_TRACE              (set) per-cycle trace of the first timeslice on stderr
+ *   LAPH_TRACE              (set) per-cycle trace of the first timeslice on stderr; "time": only
+ *                           the per-cycle filter, dense and wall times (no extra synchronisation)
  *   LAPH_NET_TARGET=1       laph_net_train_step fits the span of all K Ritz vectors of the last solve
  *                           (else its nev eigenvectors)
  *   LAPH_TEST_POISON=t      (tests) NaN written into one row of timeslice t's unlocked columns in

@@ -1222,6 +1222,7 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
   const int max_iter = b->o.max_iter > 0 ? b->o.max_iter : 60;
   int it = 0;
   for (it = 0; it < max_iter && active > 0; ++it) {
+    const auto tc0 = std::chrono::steady_clock::now();
     // one filter pass while the window settles, then T_m(B)^P per cycle; in
     // the per-cycle degree mode one pass of the degree the window allows
     // a network proposal (warm_start = 2) replaces the warm-up cycles: its
@@ -1297,6 +1298,16 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
     LAPH_TRY(orthonormalise_and_rr(v, q, it < b->qr_cycles));
     LAPH_HIP(hipEventRecord(ln->ev[2], ln->st));
     LAPH_TRY(residuals_and_window(v, lc, &active));
+    float a = 0, c = 0;
+    LAPH_HIP(hipEventElapsedTime(&a, ln->ev[0], ln->ev[1]));
+    LAPH_HIP(hipEventElapsedTime(&c, ln->ev[1], ln->ev[2]));
+    ln->ms_filter += a;
+    ln->ms_dense += c;
+    if (b->trace_time && ln->t0 == 0) {
+      const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tc0).count();
+      std::fprintf(stderr, "[laph] cycle %d time: filter %.1f ms  dense %.1f ms  wall %.1f ms  active width %d  "
+                   "unconverged timeslices %d of %d\n", it, a, c, wall, q.a, active, nt);
+    }
     if (b->trace && ln->t0 == 0) {  // per-cycle trace of the first timeslice
       const double worst = worst_residual(b, ln);
       const double* mu = &b->h_mu[0];
@@ -1322,11 +1333,6 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
       }
       std::fprintf(stderr, "\n");
     }
-    float a = 0, c = 0;
-    LAPH_HIP(hipEventElapsedTime(&a, ln->ev[0], ln->ev[1]));
-    LAPH_HIP(hipEventElapsedTime(&c, ln->ev[1], ln->ev[2]));
-    ln->ms_filter += a;
-    ln->ms_dense += c;
   }
   ln->cycles = it;
   // eigenvectors of every lane end in B[0]
@@ -1714,7 +1720,10 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     b->nev = o->nev;
     b->deg = p->deg;
     b->per_ts = (size_t)n * K;
-    b->trace = std::getenv("LAPH_TRACE") != nullptr;
+    if (const char* e = std::getenv("LAPH_TRACE")) {
+      b->trace_time = true;
+      b->trace = std::strcmp(e, "time") != 0;
+    }
     if (const char* e = std::getenv("LAPH_WARM")) b->warm_cycles = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("LAPH_WARM_DEG")) {
       b->warm_deg = std::max(0, std::atoi(e)) / 4 * 4;

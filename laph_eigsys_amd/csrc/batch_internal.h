@@ -102,6 +102,7 @@ struct laph_batch {
   int eig_mode = 0;          // Rayleigh-Ritz eigensolver: 0 zheevd, 1 zheevdj, 2 zheevj (LAPH_EIG)
   rocblas_int* jsweeps = nullptr;
   bool trace = false;       // LAPH_TRACE set: per-cycle trace of timeslice 0 on stderr
+  bool trace_time = false;  // LAPH_TRACE=time: only the per-cycle times (no extra synchronisation)
   bool have_start = false;  // B[0] holds a proposed start block (laph_net_propose)
   // vectors per block of the filter buffers; 0 until the first solve times
   // the candidates (LAPH_BW fixes it)
