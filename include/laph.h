@@ -232,6 +232,7 @@ typedef struct laph_batch laph_batch;
  *                           instead of timed on the first solve, and appended to it after timing (not
  *                           used while LAPH_BW, LAPH_STRIP, LAPH_GRAM_SPLIT, LAPH_GRAM_MFMA or
  *                           LAPH_WARM_DEG is set)
+ *   LAPH_DENSE_SUBSET=0     dense steps always on every timeslice of a lane, converged ones included
  *   LAPH_TRACE              (set) per-cycle trace of the first timeslice on stderr; "time": only
  *                           the per-cycle filter, dense and wall times (no extra synchronisation)
  *   LAPH_NET_TARGET=1       laph_net_train_step fits the span of all K Ritz vectors of the last solve
@@ -300,6 +301,9 @@ typedef struct laph_batch_stats {
                               start block DEG and up to three higher degrees (24, 32, 40, within
                               the amplification bound at the start window) are each solved and
                               timed, and the fastest is kept (LAPH_WARM_DEG fixes it) */
+  int subset_cycles;       /* cycles whose dense steps (projection, Rayleigh-Ritz, residuals) ran on
+                              the runs of unconverged timeslices only, the converged ones keeping
+                              their pairs (LAPH_DENSE_SUBSET=0 turns this off) */
 } laph_batch_stats;
 
 int laph_batch_create(laph_batch** out, const laph_params* params, const laph_batch_opts* opts);

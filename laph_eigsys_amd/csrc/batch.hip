@@ -546,12 +546,21 @@ static int rayleigh_ritz(const View& v, Act q, bool orthonormal, bool* chol_fail
   LAPH_HIP(launch_lap_residual64(b->U64 + v.lo(), T, b->mu + v.nk(), b->p.lambda_l, v.ln->wcap,
                                  fb(b, b->R32, v.vo()), b->p.L, K, nt, b->lock_blk + v.ln->t0, b->strip, b->bw,
                                  b->bf16, stencil_zrun(b), v.ln->st));
-  // the hard-locked columns of X live in the old X buffer: carry them over
-  if (q.l > 0)
-    LAPH_HIP(hipMemcpy2DAsync(T, (size_t)K * sizeof(double2), Y, (size_t)K * sizeof(double2),
-                              (size_t)q.l * sizeof(double2), (size_t)nt * b->n, hipMemcpyDeviceToDevice, v.ln->st));
-  // roles: X <- T, T <- old X buffer
-  std::swap(v.ln->iX, v.ln->iT);
+  if (v.ln->keep_roles) {
+    // dense steps on a run of the lane's timeslices: the others keep their
+    // columns in X, so the rotated active columns go back into X and the
+    // buffer roles stay
+    LAPH_HIP(hipMemcpy2DAsync(Y + q.l, (size_t)K * sizeof(double2), T + q.l, (size_t)K * sizeof(double2),
+                              (size_t)q.a * sizeof(double2), (size_t)nt * b->n, hipMemcpyDeviceToDevice, v.ln->st));
+  } else {
+    // the hard-locked columns of X live in the old X buffer: carry them over
+    if (q.l > 0)
+      LAPH_HIP(hipMemcpy2DAsync(T, (size_t)K * sizeof(double2), Y, (size_t)K * sizeof(double2),
+                                (size_t)q.l * sizeof(double2), (size_t)nt * b->n, hipMemcpyDeviceToDevice,
+                                v.ln->st));
+    // roles: X <- T, T <- old X buffer
+    std::swap(v.ln->iX, v.ln->iT);
+  }
   LAPH_HIP(hipMemcpyAsync(b->h_mu.data() + v.nk(), b->mu + v.nk(), (size_t)nt * K * sizeof(double),
                           hipMemcpyDeviceToHost, v.ln->st));
   LAPH_HIP(hipStreamSynchronize(v.ln->st));
@@ -586,6 +595,66 @@ static int orthonormalise_and_rr(const View& v, Act q, bool explicit_qr) {
     if (!failed) return LAPH_OK;
   }
   return set_error(LAPH_ERR_NOCONV, "Rayleigh-Ritz: Gram matrix not positive definite after CholQR2");
+}
+
+// Lane timeslices with an unconverged wanted pair (those the cycle's filter
+// ran on), as runs [r0, r1) of consecutive lane timeslices.
+static std::vector<std::pair<int, int>> unconverged_runs(const laph_batch* b, const laph_lane* ln) {
+  std::vector<std::pair<int, int>> runs;
+  for (int t = 0; t < ln->nt; ++t) {
+    if (b->nlock[ln->t0 + t] >= b->nev) continue;
+    if (!runs.empty() && runs.back().second == t) ++runs.back().second;
+    else runs.emplace_back(t, t + 1);
+  }
+  return runs;
+}
+
+// A strided-batched dense step costs a part independent of the batch count
+// (the latency-bound Cholesky, inverse and eigensolver chains, launches) and
+// a part per timeslice.  Measured at 32^3: the dense steps of 64 timeslices
+// take 76 ms at an active width of 64 columns (158 ms at 256), those of one
+// timeslice 18 ms, of two runs of one 34-38 ms.  The fixed part is then worth
+// about 8 (at 256 columns) to 18 (at 64) timeslices' share; the rule below
+// uses 12.
+constexpr int kDenseRunCost = 12;
+
+// Projection against the locked columns and Rayleigh-Ritz of the active
+// block.  Once some timeslices of the lane have converged (their filter no
+// longer runs, their columns stay as they are), the dense steps run on the
+// runs of unconverged timeslices alone when the runs are few enough that the
+// fixed part repeated per run costs less than the timeslices skipped.  The
+// lane's timeslice range is narrowed to each run in turn: every dense step
+// addresses its timeslices through it.
+static int dense_steps(const View& v, Act q, bool explicit_qr) {
+  laph_batch* b = v.b;
+  laph_lane* ln = v.ln;
+  const auto runs = unconverged_runs(b, ln);
+  int active = 0;
+  for (const auto& r : runs) active += r.second - r.first;
+  const bool subset = b->dense_subset && !runs.empty() && active < ln->nt &&
+                      (int)(runs.size() - 1) * kDenseRunCost < ln->nt - active;
+  if (!subset) {
+    LAPH_TRY(project_locked(v, q));
+    return orthonormalise_and_rr(v, q, explicit_qr);
+  }
+  ++ln->subset_cycles;
+  struct Restore {  // the lane's range and roles mode, also on an exception
+    laph_lane* ln;
+    int t0, nt;
+    ~Restore() {
+      ln->t0 = t0;
+      ln->nt = nt;
+      ln->keep_roles = false;
+    }
+  } restore{ln, ln->t0, ln->nt};
+  ln->keep_roles = true;
+  for (const auto& r : runs) {
+    ln->t0 = restore.t0 + r.first;
+    ln->nt = r.second - r.first;
+    LAPH_TRY(project_locked(v, q));
+    LAPH_TRY(orthonormalise_and_rr(v, q, explicit_qr));
+  }
+  return LAPH_OK;
 }
 
 // K-th lowest eigenvalue of -D on the free field (every link 1): the levels
@@ -1193,6 +1262,7 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
   ln->sv32 = ln->sv64 = ln->flaunch = 0;
   ln->recoveries = 0;
   ln->rr_fallbacks = ln->shifted = 0;
+  ln->subset_cycles = 0;
   ln->fbytes = 0;
   ln->ms_filter = ln->ms_dense = 0;
   ln->s_capture = 0;
@@ -1292,10 +1362,9 @@ static int lane_solve(laph_batch* b, laph_lane* ln, bool warm) {
     if (it == 2 && b->poison_ts >= ln->t0 && b->poison_ts < ln->t0 + nt)  // LAPH_TEST_POISON (tests)
       LAPH_HIP(hipMemsetAsync(v.X() + (size_t)(b->poison_ts - ln->t0) * b->per_ts + 7 * (size_t)K + q.l, 0xFF,
                               (size_t)q.a * sizeof(double2), ln->st));
-    LAPH_TRY(project_locked(v, q));
     // explicit CholQR2 while the block is far from orthonormal (first
     // cycles from a random start), Cholesky inside Rayleigh-Ritz after
-    LAPH_TRY(orthonormalise_and_rr(v, q, it < b->qr_cycles));
+    LAPH_TRY(dense_steps(v, q, it < b->qr_cycles));
     LAPH_HIP(hipEventRecord(ln->ev[2], ln->st));
     LAPH_TRY(residuals_and_window(v, lc, &active));
     float a = 0, c = 0;
@@ -1639,6 +1708,7 @@ static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
     s.recoveries += ln.recoveries;
     s.rr_cholqr_fallbacks += ln.rr_fallbacks;
     s.shifted_cholqr += ln.shifted;
+    s.subset_cycles += ln.subset_cycles;
     s.filter_bytes += ln.fbytes;
     s.seconds_filter += ln.ms_filter * 1e-3;
     s.seconds_dense += ln.ms_dense * 1e-3;
@@ -1756,6 +1826,7 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     }
     if (const char* e = std::getenv("LAPH_WARMUP_WINDOW_CAP")) b->warmup_cap = std::atoi(e) != 0;
     if (const char* e = std::getenv("LAPH_BLOCK_DEGREE")) b->block_degrees = std::atoi(e) != 0;
+    if (const char* e = std::getenv("LAPH_DENSE_SUBSET")) b->dense_subset = std::atoi(e) != 0;
     b->roots = laph::leja_roots(p->deg);
     b->strip = strip > 0 ? strip : p->L;
     b->strip_fixed = strip > 0;

@@ -39,6 +39,8 @@ struct laph_lane {
                                 // here so that it outlives the asynchronous uploads of the cycle
   double wcap = 0.0;            // cap of the window's lower end: 0.75 lambda_l, or the start window
                                 // in a warm-up cycle that follows a Rayleigh-Ritz start
+  bool keep_roles = false;      // dense steps on a subset of the lane's timeslices (t0, nt narrowed to
+                                // a run): the rotation is copied back into X, the roles stay
   hipGraphExec_t gplain[LAPH_MAX_PASSES - 1] = {};  // plain passes 2 .. LAPH_MAX_PASSES
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   // per-solve statistics
@@ -48,6 +50,7 @@ struct laph_lane {
   int recoveries = 0;  // timeslice blocks re-seeded after a failed orthonormalisation
   int rr_fallbacks = 0;  // Rayleigh-Ritz Gram Cholesky failures (block then orthonormalised by CholQR2)
   int shifted = 0;       // shifted Cholesky QR passes (plain CholQR Gram not numerically definite)
+  int subset_cycles = 0;  // cycles whose dense steps ran on the unconverged timeslices only
   double min_pivot2 = 0;  // LAPH_TRACE: smallest squared Cholesky pivot of the unit-diagonal Grams
                           // factored in the current cycle (CholQR and Rayleigh-Ritz), over the lane
   int min_pivot_ts = -1;  // the timeslice it belongs to
@@ -115,6 +118,8 @@ struct laph_batch {
   int warm_deg = 0;     // degree of the warm-up cycles in the per-cycle degree mode (0: the batch's degree)
   bool warm_deg_fixed = false;  // LAPH_WARM_DEG set: the warm-up degree is not timed
   bool warm_tuned = false;      // the warm-up degree was timed (or read from the layout cache)
+  bool dense_subset = true;     // dense steps on the unconverged timeslices only when that is cheaper
+                                // (LAPH_DENSE_SUBSET=0: always on the whole lane)
   bool block_degrees = true;    // per-block degrees in the per-cycle degree mode (LAPH_BLOCK_DEGREE=0: off)
   std::string tune_miss_key;    // layout cache key to append once the warm-up degree is chosen
   int strip = 0;  // sweep strip width of the stencil kernels (divides L; L = x-major planes)

@@ -66,7 +66,8 @@ class laph_batch_stats(ctypes.Structure):
                 ("filter_zrun", ctypes.c_int), ("start_window", ctypes.c_double),
                 ("rr_cholqr_fallbacks", ctypes.c_int), ("shifted_cholqr", ctypes.c_int), ("gram_blocks", ctypes.c_int),
                 ("gram_chunks", ctypes.c_int), ("gram_mfma", ctypes.c_int),
-                ("layout_from_cache", ctypes.c_int), ("warmup_degree", ctypes.c_int)]
+                ("layout_from_cache", ctypes.c_int), ("warmup_degree", ctypes.c_int),
+                ("subset_cycles", ctypes.c_int)]
 
 
 # (name, argtypes) of every entry point declared in include/laph.h

@@ -554,3 +554,36 @@ def test_block_degrees(lp, gpu, monkeypatch):
         evp = lp.recover_spectrum(theta, deg, 12.0, lamC)
         for on in (0, 1):
             np.testing.assert_allclose(runs[on][2][t], evp, rtol=1e-10, atol=1e-12, err_msg=str(on))
+
+
+def test_dense_steps_on_unconverged_timeslices(lp, gpu, monkeypatch):
+    """Once some timeslices have converged, the dense steps (projection,
+    Rayleigh-Ritz, residuals) run on the runs of unconverged timeslices
+    only (LAPH_DENSE_SUBSET=1, the default).  Six 8^3 timeslices, two of
+    them (1 and 4) from a rougher field, with the per-cycle degree held at
+    16 so that they converge a cycle after the others: every timeslice's
+    pairs against the dense eigensystem of its D (eigenvalues 1e-10,
+    residuals, orthonormality, phases), and the same spectrum as the dense
+    steps on every timeslice (LAPH_DENSE_SUBSET=0)."""
+    L, nev, deg, tol = 8, 40, 8, 1e-10
+    eps = [0.2, 0.5, 0.2, 0.2, 0.5, 0.2]
+    fields = {e: smeared(L, 1227, e, len(eps)) for e in set(eps)}
+    data = [fields[e][t] for t, e in enumerate(eps)]
+    monkeypatch.setenv("LAPH_FILTER_DEGREE", "16")
+    out = {}
+    for on in (1, 0):
+        monkeypatch.setenv("LAPH_DENSE_SUBSET", str(on))
+        b = lp.Batch(L, deg, 12.0, 0.0, len(eps), nev, nsub=64, tol=tol, seed=5, device=gpu)
+        for t, (ts, _) in enumerate(data):
+            b.set_timeslice(t, ts, 0.62, 0.62, 3)
+        st = b.solve()
+        print(f"LAPH_DENSE_SUBSET={on}: cycles {st.iterations} subset cycles {st.subset_cycles} "
+              f"max res {st.max_residual:.2e}")
+        assert st.converged == len(eps)
+        assert (st.subset_cycles >= 1) if on else (st.subset_cycles == 0)
+        out[on] = [b.get(t) for t in range(len(eps))]
+        b.close()
+    for t, (_, U) in enumerate(data):
+        ev, V, _, res = out[1][t]
+        check_pairs(ev, V, res, N.dense_D(U.reshape(L, L, L, 3, 3, 3)), nev, tol)
+        np.testing.assert_allclose(ev, out[0][t][0], rtol=1e-12, atol=1e-14)
