@@ -6,7 +6,7 @@ synthetic configuration (U = exp(i 0.3 H), HYP 0.62/0.62/3 applied on the
 device beforehand; smeared links resident in HBM when the timed region
 starts): FP32 Chebyshev filter in residual-correction form (one warm-up
 cycle of degree 16 or the higher degree the first, untimed solve found
-faster, then one filter per cycle of the degree the window allows, up to 72) on 64- or 128-vector blocks, FP64 Rayleigh-Ritz / residual
+faster, then one filter per cycle of the degree the window allows, up to 128) on 64- or 128-vector blocks, FP64 Rayleigh-Ritz / residual
 locking, until every timeslice has 220 locked pairs with
 |D v - lambda v| <= 1e-10, then fix_phase of every eigenvector: the timed
 region ends with phase-fixed eigenpairs resident in HBM.  A timed step that
@@ -241,7 +241,7 @@ def main():
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32 filter + fp64 CholQR2/RR/lock", "data": "synthetic",
         "config": {"workload": f"{L}^3x{nts} eps={EPS} HYP 0.62/0.62/3, N_ev={NEV}, Chebyshev degree {DEG}, "
-                               f"one filter pass per cycle of degree <= 72 after one of degree {st.warmup_degree}, "
+                               f"one filter pass per cycle of degree <= 128 after one of degree {st.warmup_degree}, "
                                f"vector block "
                                f"{st.filter_block}, sweep strip {st.filter_strip}, search space {NSUB}, "
                                f"residual <= {btol}, ending with phase-fixed eigenpairs in HBM",

@@ -210,7 +210,7 @@ typedef struct laph_batch laph_batch;
  *   LAPH_START_RR=1         Rayleigh-Ritz on the Gaussian start block before the first cycle (off: the
  *                           first cycle filters the block as it is)
  *   LAPH_AMP_CAP=x          bound on the amplification T_d(theta_max) (or T_DEG(theta_max)^P) of a cycle (1e10)
- *   LAPH_FILTER_DEGREE=d    highest per-cycle filter degree (72); 0: applications of T_DEG(B) per cycle
+ *   LAPH_FILTER_DEGREE=d    highest per-cycle filter degree (128); 0: applications of T_DEG(B) per cycle
  *                           (up to 5, or filter_passes)
  *   LAPH_DEGREE_SAFETY=S    a cycle's degree is lowered to the least d with T_d(theta_j) >= S r_j / tol
  *                           for every unconverged wanted pair (theta_j, r_j) of the lane (30); 0: always the
@@ -251,7 +251,7 @@ typedef struct laph_batch_opts {
                         on it, then the usual warm-up cycle); 2: the same without the warm-up cycle, the
                         proposal's Ritz values setting the first window */
   int filter_passes; /* 0: after one warm-up cycle of T_DEG(B), one Chebyshev filter per cycle of
-                        the highest degree (a multiple of 4, at most 72) whose amplification
+                        the highest degree (a multiple of 4, at most 128) whose amplification
                         T_d(theta_max) stays below 1e10; 1 .. 6: up to that many applications of
                         T_DEG(B) per cycle, fewer while T_DEG(theta_max)^P would exceed 1e10 */
   int filter_storage; /* filter vectors in HBM: 0 FP32 complex, 1 BF16 complex (half the

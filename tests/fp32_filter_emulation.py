@@ -24,7 +24,7 @@ Two filter schedules are emulated:
               before the amplification cap);
   "current":  one warm-up cycle of T_16(B) from the start window (3 x the
               K-th level of the free Laplacian), then one correction pass per
-              cycle of the highest degree d (multiple of 4, <= 72) with
+              cycle of the highest degree d (multiple of 4, <= 128) with
               T_d(theta_max) <= 1e10, applied as the product over the roots
               of (T_d(y) - T_d(theta)) / (y - theta) in their Leja order.
 
@@ -199,7 +199,7 @@ def run(schedule, L=8, K=64, nev=48, cycles=4, seed=7, eps=0.3, dtype=np.complex
             if it == 0:
                 m = 16
             else:
-                m = int(min(np.floor(np.arccosh(1e10) / np.arccosh(th_max)), 72)) // 4 * 4
+                m = int(min(np.floor(np.arccosh(1e10) / np.arccosh(th_max)), 128)) // 4 * 4
                 m = max(m, 4)
             g, w = filter_q_form(U, X, theta, c, a, m, dtype)
         amp = float(cheb(m, th_max)) ** passes
