@@ -140,11 +140,9 @@ struct laph_batch {
                              // three-term recurrence
   std::vector<double> roots; // roots of T_deg in the order the product form applies them
   int fdeg_max = 128;        // per-cycle filter degree up to this, one application per cycle after the
-                             // warm-up cycles (LAPH_FILTER_DEGREE); 0: the batch's degree and up to
-                             // filter_passes applications per cycle (an explicit filter_passes selects it).
-                             // The amplification bound is what limits a cycle at 24^3 and 32^3 (about
-                             // 72); at 48^3 it allows about 100 and the ceiling of 72 used before cost
-                             // a fifth cycle (7.00 -> 7.36 timeslices/s)
+                             // warm-up cycles (LAPH_FILTER_DEGREE; set by laph_batch_create); 0: the
+                             // batch's degree and up to filter_passes applications per cycle (an
+                             // explicit filter_passes selects it)
   // cos, sin of 2 pi k / m of the correction roots in application order, by degree m
   std::vector<std::vector<std::pair<double, double>>> qrot;
   bool qform = true;         // correction pass as a product over the roots of q_theta (LAPH_FILTER_FORM=2,
