@@ -522,7 +522,9 @@ def workload_c2(lp, args):
         dev = max(dev, float(np.max(np.abs(evp[t] - evb[t]) / np.abs(evb[t]))))
     out = {"workload": "24^3x48 eps=0.4 N_ev=120 DEG 12 tol 1e-10", "timeslices": T2,
            "batched_ts_per_s": T2 / t_batch, "batched_cycles": st.iterations,
-           "batched_warmup_degree": st.warmup_degree,
+           "batched_warmup_degree": st.warmup_degree, "batched_max_residual": st.max_residual,
+           "batched_fallbacks": {"recoveries": st.recoveries, "rr_cholqr": st.rr_cholqr_fallbacks,
+                                 "shifted_cholqr": st.shifted_cholqr},
            "plain_fp64_ts_per_s": T2 / t_plain, "plain_lambda_c": lam_c,
            "plain_restarts_max": max(restarts), "max_rel_eigenvalue_difference": dev,
            "source_shape_seconds_per_timeslice": t_shape}
@@ -593,6 +595,7 @@ def workload_c4(lp, args):
     b.solve()  # warm-up on the first group (layout autotuning, graph capture)
     sync()
     t_solve, t_set, t_store, cycles, maxres = 0.0, 0.0, 0.0, 0, 0.0
+    fallbacks = {"recoveries": 0, "rr_cholqr": 0, "shifted_cholqr": 0}
     for g0, n in groups:
         ts0 = time.perf_counter()
         for k in range(n):
@@ -604,6 +607,8 @@ def workload_c4(lp, args):
         sync()
         t_solve += time.perf_counter() - t0
         cycles, maxres = max(cycles, st.iterations), max(maxres, st.max_residual)
+        fallbacks["rr_cholqr"] += st.rr_cholqr_fallbacks
+        fallbacks["shifted_cholqr"] += st.shifted_cholqr
         assert st.converged == n and st.recoveries == 0
         t0 = time.perf_counter()
         for k in range(n):
@@ -643,7 +648,7 @@ def workload_c4(lp, args):
                       "write_GBps": wbytes / t_write / 1e9, "write_seconds_per_timeslice": t_write / 2,
                       "groups": [n for _, n in groups], "ts_per_s_solve": T4 / t_solve,
                       "ts_per_s_with_upload_hyp": T4 / (t_solve + t_set), "cycles_max": cycles,
-                      "warmup_degree": st.warmup_degree,
+                      "warmup_degree": st.warmup_degree, "fallbacks": fallbacks,
                       "max_residual": maxres, "filter_block": st.filter_block, "filter_strip": st.filter_strip,
                       "filter_GBps": by / ms / 1e6, "filter_us_per_launch": ms * 1e3, "gram": gram}))
 

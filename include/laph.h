@@ -210,6 +210,9 @@ typedef struct laph_batch laph_batch;
  *   LAPH_START_RR=1         Rayleigh-Ritz on the Gaussian start block before the first cycle (off: the
  *                           first cycle filters the block as it is)
  *   LAPH_AMP_CAP=x          bound on the amplification T_d(theta_max) (or T_DEG(theta_max)^P) of a cycle (1e10)
+ *   LAPH_WARM_AMP_CAP=x     bound on the amplification at the start window of the warm-up degrees the
+ *                           first solve times (1e12; each candidate is a whole solve, judged by the
+ *                           residual test, and one that fails is dropped)
  *   LAPH_FILTER_DEGREE=d    highest per-cycle filter degree (128); 0: applications of T_DEG(B) per cycle
  *                           (up to 5, or filter_passes)
  *   LAPH_DEGREE_SAFETY=S    a cycle's degree is lowered to the least d with T_d(theta_j) >= S r_j / tol
@@ -299,7 +302,7 @@ typedef struct laph_batch_stats {
                               cache instead of timed on the first solve */
   int warmup_degree;       /* Chebyshev degree of the warm-up cycle: on the first solve from the
                               start block DEG and up to three higher degrees (24, 32, 40, within
-                              the amplification bound at the start window) are each solved and
+                              LAPH_WARM_AMP_CAP at the start window) are each solved and
                               timed, and the fastest is kept (LAPH_WARM_DEG fixes it) */
   int subset_cycles;       /* cycles whose dense steps (projection, Rayleigh-Ritz, residuals) ran on
                               the runs of unconverged timeslices only, the converged ones keeping

@@ -1464,16 +1464,20 @@ static std::string tune_key(const laph_batch* b) {
 // Warm-up degrees the first solve times (per-cycle degree mode with a warm-up
 // cycle, LAPH_WARM_DEG unset): 24, 32 and 40 above the configuration's degree,
 // each at most the highest degree whose amplification at the start window stays
-// under amp_cap (that degree itself when it is below 40), highest first, then
-// the configuration's degree.  A higher warm-up degree can save a whole cycle
-// (24^3: 5 -> 4, 48^3: 6 -> 5) or only cost filter steps (32^3: 4 cycles at
-// every degree), so it is measured rather than set.  Empty: nothing to choose.
+// under warm_amp_cap (that degree itself when it is below 40), highest first,
+// then the configuration's degree.  A higher warm-up degree can save a whole
+// cycle (24^3: 5 -> 4, 48^3: 6 -> 5) or only cost filter steps (32^3: 4 cycles
+// at every degree), so it is measured rather than set.  The bound is wider than
+// a cycle's amp_cap: every candidate is timed as a whole solve, judged by the
+// same residual test, and one whose solve fails is dropped (at 24^3 degree 40,
+// amplification 1.3e11, saves a cycle: 91.3 -> 102.1 timeslices/s).  Empty:
+// nothing to choose.
 static std::vector<int> warm_candidates(const laph_batch* b) {
   std::vector<int> c;
   if (b->fdeg_max <= 0 || b->warm_cycles <= 0 || b->warm_deg_fixed) return c;
   const double w = b->window0, lamL = b->p.lambda_l;
   const double p0 = std::acosh(1.0 + 2.0 * w / (lamL - w));
-  int dcap = p0 > 0.0 ? (int)std::floor(std::acosh(b->amp_cap) / p0) : b->fdeg_max;
+  int dcap = p0 > 0.0 ? (int)std::floor(std::acosh(b->warm_amp_cap) / p0) : b->fdeg_max;
   dcap = std::min(dcap, b->fdeg_max) / 4 * 4;
   for (int d : {std::min(40, dcap), 32, 24})
     if (d > b->deg && d <= dcap && std::find(c.begin(), c.end(), d) == c.end()) c.push_back(d);
@@ -1667,16 +1671,29 @@ static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
     // first solve from the start block: one whole solve per warm-up degree
     // (each a valid solve), timed without the host time of first-use graph
     // captures; the fastest is kept, and solved again when it was not the last
-    // one run, so that this solve's pairs are those every later solve repeats
+    // one run, so that this solve's pairs are those every later solve repeats.
+    // One untimed solve at DEG comes first: the first solve of a batch also
+    // pays the first-use costs of the dense library kernels (code loading,
+    // workspaces), which would otherwise fall on the candidate timed first
+    // (at 24^3 they made degree 40, 12% faster once warm, look slower than 32)
+    b->warm_deg = wc.back();
+    LAPH_TRY(solve_lanes(b, false));
     double best = 0.0;
     int dbest = wc.back();
+    bool any = false;
     for (size_t i = 0; i < wc.size(); ++i) {
       b->warm_deg = wc[i];
-      LAPH_TRY(solve_lanes(b, false));
+      const int rc = solve_lanes(b, false);
+      if (rc == LAPH_ERR_NOCONV && i + 1 < wc.size()) {  // a candidate above DEG that fails is dropped
+        if (b->trace) std::fprintf(stderr, "[laph] warm-up degree %d dropped: %s\n", wc[i], laph_last_error());
+        continue;
+      }
+      LAPH_TRY(rc);
       double sec = 0.0;
       for (auto& ln : b->lanes) sec = std::max(sec, ln.s_solve - ln.s_capture);
       if (b->trace) std::fprintf(stderr, "[laph] warm-up degree %d: %.3f s\n", wc[i], sec);
-      if (i == 0 || sec < best) best = sec, dbest = wc[i];
+      if (!any || sec < best) best = sec, dbest = wc[i];
+      any = true;
     }
     b->warm_deg = dbest;
     if (dbest != wc.back()) LAPH_TRY(solve_lanes(b, false));
@@ -1818,6 +1835,7 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     if (const char* e = std::getenv("LAPH_START_RR")) b->start_rr = std::atoi(e) != 0;
     if (const char* e = std::getenv("LAPH_TRSM")) b->use_trsm = std::atoi(e) != 0;
     if (const char* e = std::getenv("LAPH_AMP_CAP")) b->amp_cap = std::max(10.0, std::atof(e));
+    if (const char* e = std::getenv("LAPH_WARM_AMP_CAP")) b->warm_amp_cap = std::max(10.0, std::atof(e));
     if (const char* e = std::getenv("LAPH_DEGREE_SAFETY")) b->deg_safety = std::max(0.0, std::atof(e));
     b->zrun = zrun;
     // first-cycle window: 3 x the free Laplacian's K-th level (measured best

@@ -148,6 +148,7 @@ struct laph_batch {
   bool qform = true;         // correction pass as a product over the roots of q_theta (LAPH_FILTER_FORM=2,
                              // the default; 1: product form reading r in every step)
   double amp_cap = 1e10;     // bound on T_m(theta_max)^P of a cycle (LAPH_AMP_CAP)
+  double warm_amp_cap = 1e12;  // bound on the warm-up degrees the first solve times (LAPH_WARM_AMP_CAP)
   double deg_safety = 30.0;  // margin S of the residual-based cycle degree (LAPH_DEGREE_SAFETY; 0: off)
   bool use_trsm = false;     // triangular solves instead of the explicit inverse in Rayleigh-Ritz (LAPH_TRSM=1)
   bool start_rr = false;     // Rayleigh-Ritz on the Gaussian start block before the first cycle (LAPH_START_RR=1)

@@ -460,9 +460,12 @@ def test_rayleigh_ritz_start_keeps_the_start_window(lp, gpu, monkeypatch):
     applications) to the plain FP64 drop-in's spectrum.  With the window taken
     from the unfiltered block's own Ritz values instead
     (LAPH_WARMUP_WINDOW_CAP=0) the warm-up filter damps only the top of the
-    spectrum and the solve needs more work."""
+    spectrum and the solve needs more work.  The warm-up degree is held at
+    DEG in all three (the first solve's timed choice could differ between
+    them)."""
     L, nts, nev, deg, tol = 16, 2, 60, 12, 1e-10
     data = smeared(L, 31, 0.4, nts)
+    monkeypatch.setenv("LAPH_WARM_DEG", str(deg))
     runs = {}
     for name, env in (("block", {"LAPH_START_RR": "0"}), ("rr", {"LAPH_START_RR": "1"}),
                       ("rr_uncapped", {"LAPH_START_RR": "1", "LAPH_WARMUP_WINDOW_CAP": "0"})):
@@ -561,15 +564,17 @@ def test_dense_steps_on_unconverged_timeslices(lp, gpu, monkeypatch):
     Rayleigh-Ritz, residuals) run on the runs of unconverged timeslices
     only (LAPH_DENSE_SUBSET=1, the default).  Six 8^3 timeslices, two of
     them (1 and 4) from a rougher field, with the per-cycle degree held at
-    16 so that they converge a cycle after the others: every timeslice's
-    pairs against the dense eigensystem of its D (eigenvalues 1e-10,
-    residuals, orthonormality, phases), and the same spectrum as the dense
-    steps on every timeslice (LAPH_DENSE_SUBSET=0)."""
+    16 and the warm-up degree at DEG so that they converge a cycle after the
+    others (6 cycles; a warm-up degree of 16 converges all six in 5): every
+    timeslice's pairs against the dense eigensystem of its D (eigenvalues
+    1e-10, residuals, orthonormality, phases), and the same spectrum as the
+    dense steps on every timeslice (LAPH_DENSE_SUBSET=0)."""
     L, nev, deg, tol = 8, 40, 8, 1e-10
     eps = [0.2, 0.5, 0.2, 0.2, 0.5, 0.2]
     fields = {e: smeared(L, 1227, e, len(eps)) for e in set(eps)}
     data = [fields[e][t] for t, e in enumerate(eps)]
     monkeypatch.setenv("LAPH_FILTER_DEGREE", "16")
+    monkeypatch.setenv("LAPH_WARM_DEG", str(deg))
     out = {}
     for on in (1, 0):
         monkeypatch.setenv("LAPH_DENSE_SUBSET", str(on))
