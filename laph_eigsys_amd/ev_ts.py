@@ -24,7 +24,7 @@ Solvers:
            group size fits the work buffers into --mem-gb of HBM, so 48^3 x 96
            streams through in groups of 24.  The first solve of a process
            times the filter and Gram layouts and the warm-up degree (several
-           seconds and up to four solves at 32^3 x 64); --tune-file keeps the
+           seconds and up to five solves at 32^3 x 64); --tune-file keeps the
            choice in a small text file (LAPH_TUNE_FILE) so that the next
            configurations of a production run read it.
 
