@@ -66,6 +66,8 @@ def main():
                     help="bound on ||D v - lambda v|| for the batched solver (the plain drop-in keeps the "
                          "reference's -eps_tol 1e-10 on T_DEG(B)); 5.8e-12 meets that criterion for every "
                          "wanted pair of the headline field")
+    ap.add_argument("--pcie-ts", type=int, default=8,
+                    help="timeslices whose eigenpairs are copied to host memory for the PCIe-inclusive rate")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-full", action="store_true",
                     help="only the CPU baseline as planned: one timeslice per host core, each solved to 1e-10 by "
@@ -193,6 +195,24 @@ def main():
     ms_launch, bytes_launch = b.time_filter(reps=args.filter_reps)
     gram = gram_rate(b, st)
     eps_res = eps_residual(lp, b, cfg, vol, sorted({0, nts - 1}), plain_lambda_c(L, NEV))
+    # PCIe-inclusive rate (never `value`): the host configuration goes in
+    # (upload, map and HYP, timed above) and every phase-fixed eigenpair comes
+    # out to host memory (laph_batch_get into one reused buffer, its pages
+    # touched first); the download is timed on a sample of timeslices
+    n_dl = max(1, min(args.pcie_ts, nts))
+    vbytes = NEV * 3 * L ** 3 * 16
+    pcie = {"unit": "timeslices/s", "upload_map_hyp_seconds": t_hyp, "download_sample_timeslices": n_dl,
+            "definition": "timeslices / (solve + upload, map and HYP of every timeslice + download of every "
+                          "timeslice's eigenpairs to host memory)"}
+    for kind in ("pageable", "pinned"):
+        with host_buffer((NEV, 3 * L ** 3), pinned=kind == "pinned") as vbuf:
+            sync()
+            t0 = time.perf_counter()
+            for t in range(n_dl):
+                b.get(t, out=vbuf)
+            t_dl = (time.perf_counter() - t0) / n_dl
+        pcie[kind] = {"value": nts / (wall / args.steps + t_hyp + nts * t_dl),
+                      "download_seconds_per_timeslice": t_dl, "download_GBps": vbytes / t_dl / 1e9}
     b.close()
     achieved = bytes_launch / (ms_launch * 1e-3)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
@@ -258,6 +278,7 @@ def main():
         "fallbacks": {"recoveries": tot["recoveries"], "rr_cholqr": tot["rr_fallbacks"],
                       "shifted_cholqr": tot["shifted"]},
         "upload_map_hyp_seconds": t_hyp,
+        "pcie_inclusive": pcie,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
@@ -670,6 +691,34 @@ def stream_ceiling():
     """JSON of laph_eigsys_amd/stream_ceiling: best GB/s of float4 streaming
     kernels for 1-3 reads + 1 write and read-only."""
     return run_measure("stream_ceiling")
+
+
+class host_buffer:
+    """A complex128 host array of the given shape, page-locked
+    (hipHostMalloc) when pinned, its pages touched before use."""
+
+    def __init__(self, shape, pinned):
+        self.shape, self.pinned, self.ptr = shape, pinned, None
+
+    def __enter__(self):
+        import ctypes
+        n = int(np.prod(self.shape))
+        if not self.pinned:
+            a = np.zeros(self.shape, dtype=np.complex128)
+            return a
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.ptr = ctypes.c_void_p()
+        assert self.hip.hipHostMalloc(ctypes.byref(self.ptr), ctypes.c_size_t(n * 16), 0) == 0
+        a = np.ctypeslib.as_array((ctypes.c_double * (2 * n)).from_address(self.ptr.value))
+        a = a.view(np.complex128).reshape(self.shape)
+        a.fill(0)
+        return a
+
+    def __exit__(self, *exc):
+        if self.ptr is not None:
+            self.hip.hipHostFree(self.ptr)
+            self.ptr = None
+        return False
 
 
 def device_sync():

@@ -424,11 +424,16 @@ class Batch:
         _check(lib().laph_batch_get_device(self.h, ts, ctypes.c_void_p(d_V), _p(ph)), "laph_batch_get_device")
         return ph
 
-    def get(self, ts, want_vectors=True):
+    def get(self, ts, want_vectors=True, out=None):
+        """Eigenvalues, phase-fixed eigenvectors (nev x n, each eigenvector
+        contiguous; written into `out` when given), phases and residuals of
+        timeslice ts."""
         ev = np.empty(self.nev)
         ph = np.empty(self.nev)
         res = np.empty(self.nev)
-        V = np.empty((self.nev, self.n), dtype=np.complex128) if want_vectors else None
+        if out is not None:
+            assert out.shape == (self.nev, self.n) and out.dtype == np.complex128 and out.flags["C_CONTIGUOUS"]
+        V = out if out is not None else (np.empty((self.nev, self.n), dtype=np.complex128) if want_vectors else None)
         _check(lib().laph_batch_get(self.h, ts, _p(ev), None if V is None else _p(V), _p(ph), _p(res)),
                "laph_batch_get")
         return ev, V, ph, res
