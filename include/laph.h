@@ -363,7 +363,10 @@ int laph_batch_time_gram(laph_batch* b, int reps, double* ms_per_gram, double* f
 /* Results of timeslice ts: eigenvalues of D (descending, i.e. nearest 0
  * first, like recover_spectrum's output), phase-fixed eigenvectors (n x nev
  * complex column-major, may be NULL), phases (may be NULL), per-pair
- * residuals |D v - lambda v| (may be NULL). */
+ * residuals |D v - lambda v| (may be NULL).  The eigenvectors pass through
+ * one n x nev device buffer that the batch allocates on first use and keeps
+ * (shared with laph_batch_write); a page-locked V_fix (hipHostMalloc) takes
+ * the direct copy (55 GB/s against about 16 GB/s into pageable memory). */
 int laph_batch_get(laph_batch* b, int ts, double* evals, double* V_fix, double* phase,
                    double* residuals);
 

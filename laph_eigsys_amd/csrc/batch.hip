@@ -1999,15 +1999,12 @@ int laph_batch_get(laph_batch* b, int ts, double* evals, double* V_fix, double* 
     for (int j = 0; j < nev; ++j) residuals[j] = std::sqrt(b->h_res[(size_t)ts * K + j]);
   if (phase)
     for (int j = 0; j < nev; ++j) phase[j] = b->h_phase[(size_t)ts * nev + j];
-  if (V_fix) {  // the solve left them phase-fixed in B[0]: only the layout changes
-    double2* dV = nullptr;
-    LAPH_HIP(hipMalloc(&dV, (size_t)b->n * nev * sizeof(double2)));
-    hipError_t e = laph::launch_extract_cols(b->B[0] + (size_t)ts * b->per_ts, dV, b->n, K, nev, b->st);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(V_fix, dV, (size_t)b->n * nev * sizeof(double2), hipMemcpyDeviceToHost, b->st);
-    if (e == hipSuccess) e = hipStreamSynchronize(b->st);
-    (void)hipFree(dV);
-    LAPH_HIP(e);
+  if (V_fix) {  // the solve left them phase-fixed in B[0]: only the layout changes, in the
+                // batch's output buffer (allocated once, shared with laph_batch_write)
+    LAPH_TRY(laph::output_buffers(b));
+    LAPH_HIP(laph::launch_extract_cols(b->B[0] + (size_t)ts * b->per_ts, b->wV, b->n, K, nev, b->st));
+    LAPH_HIP(hipMemcpyAsync(V_fix, b->wV, (size_t)b->n * nev * sizeof(double2), hipMemcpyDeviceToHost, b->st));
+    LAPH_HIP(hipStreamSynchronize(b->st));
   }
   return LAPH_OK;
 }
