@@ -76,6 +76,14 @@ def test_batch_set_links_matches_set_timeslice(lp, gpu):
     b2.set_timeslice(0, ts, 0.62, 0.62, 3)
     b2.solve()
     ev2, V2, _, _ = b2.get(0)
+    # the eigenvectors into a caller's buffer, twice (the batch's staging
+    # buffer is reused): the same values, in the caller's array
+    out = np.full_like(V2, np.nan)
+    for _ in range(2):
+        ev3, V3, _, _ = b2.get(0, out=out)
+        assert V3 is out
+        np.testing.assert_array_equal(V3, V2)
+        np.testing.assert_array_equal(ev3, ev2)
     np.testing.assert_allclose(ev1, ev2, rtol=1e-12)
 
 
