@@ -24,9 +24,10 @@ Solvers:
            group size fits the work buffers into --mem-gb of HBM, so 48^3 x 96
            streams through in groups of 24.  The first solve of a process
            times the filter and Gram layouts and the warm-up degree (several
-           seconds and up to five solves at 32^3 x 64); --tune-file keeps the
-           choice in a small text file (LAPH_TUNE_FILE) so that the next
-           configurations of a production run read it.
+           seconds and up to five solves at 32^3 x 64); the choice is kept in
+           a small text file (LAPH_TUNE_FILE; by default laph_layouts.txt in
+           the result directory, --tune-file to move it, 'none' to turn it
+           off) so that the next configurations of a production run read it.
 
 The reference divides LT over MPI ranks (ev_ts.cpp:96-126); one process
 here takes [tstart, tend] (default: all of LT), the share one rank would get.
@@ -214,7 +215,8 @@ def run(pars, config, solver="batched", tstart=0, tend=None, group=0, mem_gb=200
             dt = time.perf_counter() - tp
             out["solve_seconds"] += dt
             out["groups"].append({"tstart": tstart + g0, "timeslices": gn, "seconds": dt, "cycles": st.iterations,
-                                  "max_residual": st.max_residual})
+                                  "max_residual": st.max_residual, "layout_from_cache": st.layout_from_cache,
+                                  "warmup_degree": st.warmup_degree})
             log(f"group t={tstart + g0}..{tstart + g0 + gn - 1}: {dt:.2f} s, cycles {st.iterations}, "
                 f"max residual {st.max_residual:.2e}")
             tw = time.perf_counter()
@@ -225,6 +227,20 @@ def run(pars, config, solver="batched", tstart=0, tend=None, group=0, mem_gb=200
         if b is not None:
             b.close()
     return out
+
+
+def tune_file_path(arg, pars, env):
+    """Layout cache of a run: --tune-file when given ('none' turns the cache
+    off), else LAPH_TUNE_FILE when set, else laph_layouts.txt in the result
+    directory, so that the configurations of one production run (one process
+    each, like the reference's) time the layouts and the warm-up degree once."""
+    if arg == "none":
+        return None
+    if arg:
+        return arg
+    if env.get("LAPH_TUNE_FILE"):
+        return env["LAPH_TUNE_FILE"]
+    return os.path.join(pars.get_path("res"), "laph_layouts.txt")
 
 
 def main(argv=None):
@@ -242,12 +258,16 @@ def main(argv=None):
     ap.add_argument("--tol", type=float, default=1e-10, help="residual |D v - lambda v| (batched), EPS tol (plain)")
     ap.add_argument("--device", type=int, default=0)
     ap.add_argument("--json", default=None, help="write the timing summary here")
-    ap.add_argument("--tune-file", default=None,
-                    help="layout cache of the batched solver (read, and appended after timing a new shape)")
+    ap.add_argument("--tune-file", default="",
+                    help="layout cache of the batched solver (read, and appended after timing a new shape); "
+                         "default <result_path>/laph_layouts.txt, 'none': time the layouts in every run")
     args = ap.parse_args(argv)
-    if args.tune_file:
-        os.environ["LAPH_TUNE_FILE"] = args.tune_file
     pars = IO().set_values(args.params)
+    tune = tune_file_path(args.tune_file, pars, os.environ)
+    if tune:
+        os.environ["LAPH_TUNE_FILE"] = tune
+    else:
+        os.environ.pop("LAPH_TUNE_FILE", None)
     pars.print_summary()
     print(f"calculating config {args.conf}")
     out = run(pars, args.conf, args.solver, args.tstart, args.tend, args.group, args.mem_gb, args.nsub, args.tol,

@@ -35,6 +35,18 @@ def test_parameter_file(tmp_path):
     assert p.get_path("phase_path") == "./old"
 
 
+def test_tune_file_default_is_in_the_result_directory(tmp_path):
+    """The layout cache of a run: --tune-file wins ('none' turns it off),
+    then LAPH_TUNE_FILE, else laph_layouts.txt beside the results."""
+    f = tmp_path / "parameters.txt"
+    f.write_text(PARAMS)
+    p = ev_ts.IO().set_values(str(f))
+    assert ev_ts.tune_file_path("", p, {}) == "./out/laph_layouts.txt"
+    assert ev_ts.tune_file_path("", p, {"LAPH_TUNE_FILE": "/x/t.txt"}) == "/x/t.txt"
+    assert ev_ts.tune_file_path("/y/u.txt", p, {"LAPH_TUNE_FILE": "/x/t.txt"}) == "/y/u.txt"
+    assert ev_ts.tune_file_path("none", p, {"LAPH_TUNE_FILE": "/x/t.txt"}) is None
+
+
 def test_groups_cover_range_and_fit():
     # 48^3 x 96, nev 200: 6.8 GB per timeslice -> 4 groups of 24 under 200 GB
     g = ev_ts.plan_groups(96, 48, 200, 200.0)

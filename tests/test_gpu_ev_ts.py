@@ -50,6 +50,36 @@ def test_driver_files(gpu, tmp_path):
         np.testing.assert_allclose(results[("plain", t)], results[("batched", t)], rtol=1e-10, atol=1e-12)
 
 
+def test_driver_layout_cache_in_result_directory(gpu, tmp_path, monkeypatch):
+    """The per-configuration program keeps the batched solver's layouts and
+    warm-up degree in <result_path>/laph_layouts.txt by default: a second
+    process-like run of the same shape reads them instead of timing, and
+    writes the same eigensystem files bit for bit."""
+    import json
+    from laph_eigsys_amd import lapheig as lp
+    monkeypatch.setenv("LAPH_TUNE_FILE", "unset-below")  # restored after the test
+    monkeypatch.delenv("LAPH_TUNE_FILE")
+    L, LT, nev, deg, conf = 8, 2, 20, 8, 31
+    (tmp_path / "confs").mkdir()
+    lp.write_lime(str(tmp_path / "confs" / f"conf.{conf:04d}"), N.synth_config(5, 0.4, L, LT), LT, L)
+    out = tmp_path / "res"
+    pf = tmp_path / "parameters.txt"
+    pf.write_text(f"config_path = {tmp_path / 'confs'}\nresult_path = {out}\nLT = {LT}\nLX = {L}\nNEV = {nev}\n"
+                  f"alpha_1 = 0.62\nalpha_2 = 0.62\niter = 3\nLAM_L = 12.0\nLAM_C = 1.0\nDEG = {deg}\n")
+    runs = []
+    for k in range(2):
+        js = tmp_path / f"run{k}.json"
+        assert ev_ts.main(["--conf", str(conf), "--params", str(pf), "--json", str(js), "--device", str(gpu)]) == 0
+        info = json.loads(js.read_text())
+        files = {t: (out / f"eigenvectors.{conf:04d}.{t:03d}").read_bytes() for t in range(LT)}
+        runs.append((info["groups"][0], files))
+    assert (out / "laph_layouts.txt").exists()
+    assert runs[0][0]["layout_from_cache"] == 0 and runs[1][0]["layout_from_cache"] == 1
+    assert runs[0][0]["warmup_degree"] == runs[1][0]["warmup_degree"]
+    for t in range(LT):
+        assert runs[0][1][t] == runs[1][1][t]
+
+
 def test_batch_write_matches_host_path(gpu, tmp_path):
     """laph_batch_write (device phase fix, device V^H V guard, chunked
     pinned streaming) writes the same bytes as laph_batch_get followed by the
