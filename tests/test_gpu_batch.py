@@ -600,3 +600,45 @@ def test_dense_steps_on_unconverged_timeslices(lp, gpu, monkeypatch):
         ev, V, _, res = out[1][t]
         check_pairs(ev, V, res, N.dense_D(U.reshape(L, L, L, 3, 3, 3)), nev, tol)
         np.testing.assert_allclose(ev, out[0][t][0], rtol=1e-12, atol=1e-14)
+
+
+def test_pairs_meet_the_eps_criterion_at_the_derived_bound(lp, gpu):
+    """SLEPc's relative test on T_DEG(B), |T_DEG(B) v - theta v| / |theta| <=
+    1e-10 (the plain drop-in's -eps_tol), follows for every pair from a bound
+    on |D v - lambda v| of 1e-10 / max_j c |T'_DEG(b_j)| / |T_DEG(b_j)|,
+    b_j = c lambda_j + a (first order in the residual): the batched solver run
+    at that bound, every pair's relative residual computed through the
+    MatShell entry point."""
+    L, nts, nev, deg = 12, 2, 40, 8
+    data = smeared(L, 4242, 0.35, nts)
+
+    def solve(tol):
+        b = lp.Batch(L, deg, 12.0, 0.0, nts, nev, nsub=64, tol=tol, seed=3, device=gpu)
+        for t, (ts, _) in enumerate(data):
+            b.set_timeslice(t, ts, 0.62, 0.62, 3)
+        st = b.solve()
+        assert st.converged == nts
+        out = [b.get(t) for t in range(nts)]
+        b.close()
+        return out
+
+    first = solve(1e-10)
+    lam_c = 1.3 * max(-ev[-1] for ev, _, _, _ in first)
+    c = 2.0 / (12.0 - lam_c)
+    a = 1.0 + c * lam_c
+
+    def factor(lam):  # c |T'_m(b)| / |T_m(b)|, b = cosh(psi) > 1
+        psi = np.arccosh(c * lam + a)
+        return c * deg * np.tanh(deg * psi) / np.sinh(psi)
+
+    fmax = max(float(np.max(factor(ev))) for ev, _, _, _ in first)
+    tol_d = 1e-10 / fmax
+    print(f"lambda_c {lam_c:.4f}: largest factor {fmax:.2f}, bound on |D v - lambda v| {tol_d:.2e}")
+    for t, (ev, V, _, res) in enumerate(solve(tol_d)):
+        assert res.max() <= tol_d
+        theta = np.cosh(deg * np.arccosh(c * ev + a))
+        with lp.Tslice(L, deg, 12.0, lam_c, device=gpu, ts_config=data[t][0]) as tsl:
+            tsl.smearing_hyp(0.62, 0.62, 3)
+            rel = [np.linalg.norm(tsl.MatMult_Laplacian2D(V[j]) - theta[j] * V[j]) / abs(theta[j]) for j in range(nev)]
+        print(f"timeslice {t}: largest relative EPS residual {max(rel):.2e}")
+        assert max(rel) <= 1e-10
