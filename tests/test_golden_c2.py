@@ -70,3 +70,37 @@ def test_c2_plain_and_batched(gpu):
         check(ev, V, t)
         assert lp.check_trace(V)[0]
     b.close()
+
+
+@pytest.mark.gpu
+def test_c2_every_timeslice_batched_equals_plain(gpu):
+    """All 48 timeslices of C2 in one batch (the configuration's whole
+    lattice, as bench.py --workload c2 runs it) against the plain FP64
+    drop-in timeslice by timeslice: eigenvalues 1e-10 relative, every pair's
+    residual within the bound, and the fixture's timeslices 0 and 47 as
+    above."""
+    from laph_eigsys_amd import lapheig as lp
+    cfg = lp.synth_config(int(G["seed"]), float(G["eps"]), L, T)
+    vol = 72 * L ** 3
+    b = lp.Batch(L, DEG, LAM_L, 0.0, T, NEV, tol=1e-10, seed=5, device=gpu)
+    for t in range(T):
+        b.set_timeslice(t, cfg[t * vol:(t + 1) * vol], 0.62, 0.62, 3)
+    st = b.solve()
+    assert st.converged == T and st.max_residual <= 1e-10
+    evb = [b.get(t, want_vectors=False) for t in range(T)]
+    b.close()
+    print(f"C2 batch of {T}: cycles {st.iterations}, warm-up degree {st.warmup_degree}, max res {st.max_residual:.2e}")
+    lam_c = 1.35 * max(-ev[-1] for ev, _, _, _ in evb)
+    worst = 0.0
+    for t in range(T):
+        ev, _, _, res = evb[t]
+        assert res.max() <= 1e-10
+        with lp.Tslice(L, DEG, LAM_L, lam_c, device=gpu, ts_config=cfg[t * vol:(t + 1) * vol]) as sl:
+            sl.smearing_hyp(0.62, 0.62, 3)
+            theta, _, _ = sl.eps_solve(NEV, tol=1e-10, want_vectors=False)
+        evp = lp.recover_spectrum(theta, DEG, LAM_L, lam_c)
+        np.testing.assert_allclose(ev, evp, rtol=1e-10, atol=0, err_msg=str(t))
+        worst = max(worst, float(np.max(np.abs(ev - evp) / np.abs(evp))))
+        if t in TSL:
+            check(ev, None, t)
+    print(f"largest relative eigenvalue difference batched vs plain over {T} timeslices: {worst:.2e}")
