@@ -211,6 +211,7 @@ def main():
             for t in range(n_dl):
                 b.get(t, out=vbuf)
             t_dl = (time.perf_counter() - t0) / n_dl
+            del vbuf  # the pinned memory is freed when the block ends
         pcie[kind] = {"value": nts / (wall / args.steps + t_hyp + nts * t_dl),
                       "download_seconds_per_timeslice": t_dl, "download_GBps": vbytes / t_dl / 1e9}
     b.close()
