@@ -241,7 +241,9 @@ typedef struct laph_batch laph_batch;
  *   LAPH_NET_TARGET=1       laph_net_train_step fits the span of all K Ritz vectors of the last solve
  *                           (else its nev eigenvectors)
  *   LAPH_TEST_POISON=t      (tests) NaN written into one row of timeslice t's unlocked columns in
- *                           cycle 2, to exercise the recovery counted in laph_batch_stats */
+ *                           cycle 2, to exercise the recovery counted in laph_batch_stats
+ *   LAPH_TEST_FAIL_WARM=d   (tests) the first solve's warm-up degree candidate d reports a failure
+ *                           without solving, to exercise the dropping of a failed candidate */
 
 typedef struct laph_batch_opts {
   int nts;           /* timeslices held by the batch */

@@ -1683,7 +1683,9 @@ static int solve_impl(laph_batch* b, laph_batch_stats* stats) {
     bool any = false;
     for (size_t i = 0; i < wc.size(); ++i) {
       b->warm_deg = wc[i];
-      const int rc = solve_lanes(b, false);
+      const int rc = b->test_fail_warm == wc[i] && i + 1 < wc.size()  // LAPH_TEST_FAIL_WARM (tests)
+                         ? set_error(LAPH_ERR_NOCONV, "warm-up degree %d failed (LAPH_TEST_FAIL_WARM)", wc[i])
+                         : solve_lanes(b, false);
       if (rc == LAPH_ERR_NOCONV && i + 1 < wc.size()) {  // a candidate above DEG that fails is dropped
         if (b->trace) std::fprintf(stderr, "[laph] warm-up degree %d dropped: %s\n", wc[i], laph_last_error());
         continue;
@@ -1822,6 +1824,7 @@ int laph_batch_create(laph_batch** out, const laph_params* p, const laph_batch_o
     b->force_shift = std::getenv("LAPH_SHIFTED_CHOLQR") != nullptr;
     if (const char* e = std::getenv("LAPH_QR_CYCLES")) b->qr_cycles = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("LAPH_TEST_POISON")) b->poison_ts = std::atoi(e);
+    if (const char* e = std::getenv("LAPH_TEST_FAIL_WARM")) b->test_fail_warm = std::atoi(e);
     if (const char* e = std::getenv("LAPH_FILTER_FORM")) {
       b->prod = std::atoi(e) != 0;
       b->qform = std::atoi(e) == 2;

@@ -128,6 +128,7 @@ struct laph_batch {
   bool tuned = false;        // filter layout chosen (first solve)
   bool tuned_from_cache = false;  // ... read from the LAPH_TUNE_FILE layout cache instead of timed
   int poison_ts = -1;  // LAPH_TEST_POISON: NaN into one row of this timeslice's active block at cycle 2 (tests)
+  int test_fail_warm = -1;  // LAPH_TEST_FAIL_WARM: this warm-up degree candidate reports a failure (tests)
   int qr_cycles = 0;  // cycles (and the start block) orthonormalised by explicit CholQR2 (LAPH_QR_CYCLES)
   // laph_batch_write: phase-fixed copy in the file layout, V^H V, rocBLAS handle on st,
   // two pinned staging chunks and their events (created on first use)

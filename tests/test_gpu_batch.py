@@ -6,6 +6,8 @@ and the plain FP64 path.
 Bar: eigenvalues within 1e-10 relative of the FP64 oracle, per-pair
 residual |D v - lambda v| <= tol, tr(V^H V) = nev, phase-fixed eigenvectors
 within 1e-9 per component where the eigenvalue gap exceeds 1e-4."""
+import os
+
 import numpy as np
 import pytest
 
@@ -533,6 +535,18 @@ def test_warmup_degree_timed_on_first_solve(lp, gpu, monkeypatch):
     b.close()
     assert st3.warmup_degree == 24 and st3.converged == nts
     for a, c in zip(ev1, ev3):
+        np.testing.assert_allclose(c, a, rtol=1e-10, atol=1e-12)
+    # a candidate above DEG whose solve fails is dropped: the first solve
+    # keeps another degree and returns its converged pairs
+    monkeypatch.delenv("LAPH_WARM_DEG")
+    monkeypatch.setenv("LAPH_TEST_FAIL_WARM", str(st1.warmup_degree if st1.warmup_degree != deg else 24))
+    b = batch()
+    st4 = b.solve()
+    ev4 = [b.get(t, want_vectors=False)[0] for t in range(nts)]
+    b.close()
+    print(f"failed candidate {int(os.environ['LAPH_TEST_FAIL_WARM'])}: kept {st4.warmup_degree}")
+    assert st4.converged == nts and st4.warmup_degree != int(os.environ["LAPH_TEST_FAIL_WARM"])
+    for a, c in zip(ev1, ev4):
         np.testing.assert_allclose(c, a, rtol=1e-10, atol=1e-12)
 
 
