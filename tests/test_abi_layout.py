@@ -1,6 +1,6 @@
-"""The ctypes mirrors of the C-ABI structs (laph_eigsys_amd/lapheig.py)
-against include/laph.h as a C compiler lays them out: every struct's size
-and every field's offset.  The library writes whole structs into the
+"""The ctypes mirrors of the C ABI (laph_eigsys_amd/lapheig.py) against
+include/laph.h: every struct's size and field offsets as a C compiler lays
+them out, and every entry point's argument kinds.  The library writes whole structs into the
 caller's memory (laph_batch_solve fills laph_batch_stats), so a field added
 on one side only would write past the Python buffer.  No GPU: the C side is
 a host program compiled from the header alone."""
@@ -61,3 +61,39 @@ def test_header_fields_all_mirrored():
         body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
         members = re.findall(r"\b(\w+)\s*;", body)
         assert members == [f for f, _ in getattr(lapheig, s)._fields_], s
+
+
+def _kind_c(decl):
+    decl = " ".join(decl.replace("const ", "").split())
+    if "*" in decl:
+        return "ptr"
+    base = decl.rsplit(" ", 1)[0]
+    return {"int": "int", "double": "double", "unsigned long long": "u64"}[base]
+
+
+def _kind_ctypes(t):
+    if t in (ctypes.c_int,):
+        return "int"
+    if t in (ctypes.c_double,):
+        return "double"
+    if t in (ctypes.c_ulonglong,):
+        return "u64"
+    if t in (ctypes.c_void_p, ctypes.c_char_p) or issubclass(t, ctypes._Pointer):
+        return "ptr"
+    raise AssertionError(t)
+
+
+def test_ctypes_signatures_match_the_prototypes():
+    """Every prototype of include/laph.h against lapheig.SIGNATURES: the same
+    entry points, the same number of arguments and, per argument, the same
+    kind (int, double, 64-bit unsigned, pointer); int or string results."""
+    import re
+    hdr = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "laph.h")).read(), flags=re.S)
+    protos = re.findall(r"\b(int|const char\s*\*)\s+(laph_\w+)\s*\(([^)]*)\)\s*;", hdr)
+    assert len(protos) >= 40
+    assert {n for _, n, _ in protos} == set(lapheig.SIGNATURES)
+    for ret, name, args in protos:
+        argtypes, restype = lapheig.SIGNATURES[name]
+        c_args = [] if args.strip() == "void" else [_kind_c(a) for a in args.split(",")]
+        assert c_args == [_kind_ctypes(t) for t in argtypes], name
+        assert restype is (ctypes.c_char_p if "char" in ret else ctypes.c_int), name
