@@ -436,6 +436,15 @@ def summarize_mfma(path):
         print(f"{name:70s} {n:6d} {ns / 1e6:9.2f} {fl / 1e12:9.3f} {rate:8.2f} {rate / (PEAK_FP64_MFMA / 1e12):8.3f}")
 
 
+# kernels of the dense phase whose counter traffic --summarize-pmc reports
+# beside the filter's: label and the name fragments (demangled, as the
+# counter files give them, or mangled) that identify them
+_DENSE_KERNELS = {"k_lap64y (D X)": ("k_lap64y<false", "k_lap64yILb0"),
+                  "k_lap64y (residual)": ("k_lap64y<true", "k_lap64yILb1"),
+                  "k_combine_gamma": ("k_combine_gamma",), "k_residual_norms": ("k_residual_norms",),
+                  "k_gram_mfma": ("k_gram_mfma",)}
+
+
 def summarize_pmc(fetch_csv, write_csv, last):
     """HBM bytes per isolated filter launch: mean FETCH_SIZE and WRITE_SIZE
     (KiB per dispatch) over the last `last` filter dispatches, each scaled by
@@ -457,9 +466,28 @@ def summarize_pmc(fetch_csv, write_csv, last):
     w, wc, nw = read(write_csv, "WRITE_SIZE")
     kf, kw = (nbytes / 1024) / fc, (nbytes / 1024) / wc
     traffic = (f * kf + w * kw) * 1024
+    # the dense phase's own kernels in the same runs (the solve before the
+    # replays): scaled bytes read and written per dispatch, median and largest
+    # (the largest are the full-width dispatches); the two runs are separate
+    # processes, so reads and writes are summarised apart
+    others = {}
+    for counter, path, scale, key in (("FETCH_SIZE", fetch_csv, kf, "read"), ("WRITE_SIZE", write_csv, kw, "write")):
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            name = next((k for k, frags in _DENSE_KERNELS.items() if any(f in r["Kernel_Name"] for f in frags)),
+                        None)
+            if name:
+                others.setdefault(name, {}).setdefault(key, []).append(float(r["Counter_Value"]) * scale * 1024)
+    dense = {}
+    for name, d in sorted(others.items()):
+        dense[name] = {}
+        for key, v in d.items():
+            v.sort()
+            dense[name][key] = {"dispatches": len(v), "median_bytes": v[len(v) // 2], "max_bytes": v[-1]}
     print(json.dumps({"filter_dispatches": [nf, nw], "fetch_KiB": f, "write_KiB": w,
                       "copy_fetch_KiB": fc, "copy_write_KiB": wc, "fetch_scale": kf, "write_scale": kw,
-                      "traffic_bytes_per_launch": traffic}))
+                      "traffic_bytes_per_launch": traffic, "dense_kernels": dense}))
 
 
 def workload_fields(lp, args):

@@ -44,8 +44,9 @@ def test_traffic_calibration(bench, tmp_path, capsys):
     # 1 GiB copy calibrates both, so 3 MiB fetched and 2 MiB written per
     # filter dispatch become 6 MiB + 2 MiB of traffic
     gib_kib = (1 << 30) / 1024
-    fetch = [("k_filter_step32x2", 3 * 1024.0)] * 4 + [("__amd_rocclr_copyBuffer", gib_kib / 2)]
-    write = [("k_filter_step32x2", 2 * 1024.0)] * 4 + [("__amd_rocclr_copyBuffer", gib_kib)]
+    dense = [("_ZN4laph8k_lap64yILb0ELi64E", 1024.0), ("_ZN4laph8k_lap64yILb0ELi64E", 4096.0)]
+    fetch = dense + [("k_filter_step32x2", 3 * 1024.0)] * 4 + [("__amd_rocclr_copyBuffer", gib_kib / 2)]
+    write = dense + [("k_filter_step32x2", 2 * 1024.0)] * 4 + [("__amd_rocclr_copyBuffer", gib_kib)]
     _pmc_csv(tmp_path / "f.csv", "FETCH_SIZE", fetch)
     _pmc_csv(tmp_path / "w.csv", "WRITE_SIZE", write)
     bench.summarize_pmc(str(tmp_path / "f.csv"), str(tmp_path / "w.csv"), 4)
@@ -53,6 +54,10 @@ def test_traffic_calibration(bench, tmp_path, capsys):
     assert out["filter_dispatches"] == [4, 4]
     assert out["fetch_scale"] == pytest.approx(2.0) and out["write_scale"] == pytest.approx(1.0)
     assert out["traffic_bytes_per_launch"] == pytest.approx(8 * 1024 * 1024)
+    # the dense phase's kernels of the same runs, scaled like the filter's
+    dx = out["dense_kernels"]["k_lap64y (D X)"]
+    assert dx["read"]["dispatches"] == 2 and dx["read"]["max_bytes"] == pytest.approx(2 * 4096 * 1024)
+    assert dx["write"]["max_bytes"] == pytest.approx(4096 * 1024)
 
 
 def test_mfma_rate(bench, tmp_path, capsys):
