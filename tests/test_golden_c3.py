@@ -6,11 +6,12 @@ oracle's Krylov-Schur; written by tests/golden/make_c3.py), timeslices 0
 and 63.
 
 The batched solver runs exactly as bench.py times it: all 64 timeslices in
-one batch, search space 256, filter degree 16 for the warm-up cycle and up
-to 72 after it, the filter layout picked by the autotuner on the first solve.
-It is solved twice; both solves must match the fixture and each other bit
-for bit.  The plain FP64 drop-in (the MatShell path) solves the same two
-timeslices with the lambda_c rule bench.py uses.
+one batch, search space 256, the warm-up degree and the filter layout
+picked on the first solve, then per-cycle degrees up to 128 (72 here, where
+the amplification bound stops them).  It is solved twice; both solves must
+match the fixture and each other bit for bit.  The plain FP64 drop-in (the
+MatShell path) solves the same two timeslices with the lambda_c rule
+bench.py uses, and every one of the 64 timeslices for the eigenvalues.
 
 Bar: eigenvalues 1e-10 relative (ev_ts.cpp:263-303 with recover_spectrum,
 recover_spec.cpp:45-54, for the plain path); phase-fixed leading components
@@ -88,6 +89,8 @@ def test_c3_batched_at_bench_settings_and_plain(gpu):
                   f"{st.rr_cholqr_fallbacks} shifted {st.shifted_cholqr} recoveries {st.recoveries}", flush=True)
             assert st.converged == T and st.max_residual <= 1e-10 and st.recoveries == 0
             got.append({t: b.get(t) for t in TSL})
+            if rep == 0:
+                ev_all = [b.get(t, want_vectors=False)[0] for t in range(T)]
         for t in TSL:
             ev, V, ph, res = got[0][t]
             check(ev, V, t)
@@ -107,3 +110,13 @@ def test_c3_batched_at_bench_settings_and_plain(gpu):
         print(f"C3 plain t={t}: restarts {pst.restarts} matvecs {pst.matvecs} {pst.seconds:.2f}s", flush=True)
         ev = lp.recover_spectrum(theta, DEG, LAM_L, lam_c)
         check(ev, V, t)
+    # every timeslice of the headline batch against the plain drop-in
+    worst = 0.0
+    for t in range(T):
+        with lp.Tslice(L, DEG, LAM_L, lam_c, device=gpu, ts_config=cfg[t * vol:(t + 1) * vol]) as sl:
+            sl.smearing_hyp(0.62, 0.62, 3)
+            theta, _, _ = sl.eps_solve(NEV, tol=1e-10, want_vectors=False)
+        ev = lp.recover_spectrum(theta, DEG, LAM_L, lam_c)
+        np.testing.assert_allclose(ev_all[t], ev, rtol=1e-10, atol=0, err_msg=str(t))
+        worst = max(worst, float(np.max(np.abs(ev_all[t] - ev) / np.abs(ev))))
+    print(f"C3: largest relative eigenvalue difference batched vs plain over {T} timeslices {worst:.2e}", flush=True)
